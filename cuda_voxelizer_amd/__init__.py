@@ -1,0 +1,235 @@
+"""MI355X-native mesh voxelizer: Python host mirror of the reference API.
+
+The reference (Forceflow/cuda_voxelizer v0.6) exposes, in C++:
+
+* ``AABox<T>`` and ``voxinfo``                          (src/util.h:41-69)
+* ``createMeshBBCube(AABox<float3>)``                   (src/util.h:80-110)
+* ``voxelize(const voxinfo&, float*, unsigned int*, bool morton)``
+                                                        (src/voxelize.cu:192-238)
+* ``voxelize_solid(const voxinfo&, float*, unsigned int*, bool morton)``
+                                                        (src/voxelize_solid.cu:147-193)
+
+This module mirrors those names and argument meanings over the C-ABI of
+``libvoxelizer_amd.so`` (hand-written HIP for gfx950).  Device buffers are
+torch tensors on a HIP device; torch is used only for allocation and streams.
+``voxelize``/``voxelize_solid`` keep the reference semantics: synchronous, and
+OR/XOR into whatever the table holds.  ``voxelize_surface``/``voxelize_solid_table``
+are the stream-ordered forms that define the whole table (fused zero-fill).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _capi
+from ._capi import (VOX_FLAG_ACCUMULATE, VOX_FLAG_ALGO_DIRECT, VOX_LAYOUT_LINEAR, VOX_LAYOUT_MORTON,
+                    VoxError, check, vox_info)
+
+__all__ = [
+    "AABox", "VoxInfo", "create_mesh_bbox_cube", "createMeshBBCube", "voxelize", "voxelize_solid",
+    "voxelize_surface", "voxelize_solid_table", "table_bytes", "table_words", "mesh_bbox",
+    "voxinfo_for_mesh", "VoxError", "VOX_LAYOUT_LINEAR", "VOX_LAYOUT_MORTON", "VOX_FLAG_ACCUMULATE",
+    "VOX_FLAG_ALGO_DIRECT",
+]
+
+
+@dataclass
+class AABox:
+    """Axis-aligned box (util.h:41-47); min/max are float32[3]."""
+    min: np.ndarray
+    max: np.ndarray
+
+    def __post_init__(self):
+        self.min = np.asarray(self.min, dtype=np.float32).reshape(3).copy()
+        self.max = np.asarray(self.max, dtype=np.float32).reshape(3).copy()
+
+
+class VoxInfo:
+    """Mirror of the reference ``voxinfo`` (util.h:50-69).
+
+    ``VoxInfo(bbox, gridsize, n_triangles)`` computes
+    ``unit = (bbox.max - bbox.min) / float(gridsize)`` in binary32 (util.h:56-61)
+    through the library, so host and kernels share one definition."""
+
+    def __init__(self, bbox: AABox, gridsize, n_triangles: int):
+        g = np.asarray(gridsize, dtype=np.int64).reshape(-1)
+        if g.size == 3:
+            if not (g[0] == g[1] == g[2]):
+                raise ValueError("only cubic grids are supported (the reference builds cubic grids)")
+            g = int(g[0])
+        else:
+            g = int(g[0])
+        lib = _capi.load()
+        self._c = vox_info()
+        check(lib.vox_make_info(_f3(bbox.min), _f3(bbox.max), g, int(n_triangles), ctypes.byref(self._c)))
+
+    @classmethod
+    def _from_struct(cls, s: vox_info) -> "VoxInfo":
+        obj = cls.__new__(cls)
+        obj._c = s
+        return obj
+
+    @property
+    def bbox(self) -> AABox:
+        return AABox(np.array(self._c.bbox_min[:], np.float32), np.array(self._c.bbox_max[:], np.float32))
+
+    @property
+    def gridsize(self) -> tuple[int, int, int]:
+        return tuple(int(v) for v in self._c.gridsize)
+
+    @property
+    def grid(self) -> int:
+        return int(self._c.gridsize[0])
+
+    @property
+    def n_triangles(self) -> int:
+        return int(self._c.n_triangles)
+
+    @n_triangles.setter
+    def n_triangles(self, n: int) -> None:
+        self._c.n_triangles = int(n)
+
+    @property
+    def unit(self) -> np.ndarray:
+        return np.array(self._c.unit[:], dtype=np.float32)
+
+    def as_bytes(self) -> bytes:
+        return ctypes.string_at(ctypes.addressof(self._c), 64)
+
+    def __repr__(self) -> str:
+        b = self.bbox
+        return (f"VoxInfo(bbox=({b.min.tolist()}, {b.max.tolist()}), grid={self.gridsize}, "
+                f"n_triangles={self.n_triangles}, unit={self.unit.tolist()})")
+
+
+def _f3(a) -> ctypes.Array:
+    a = np.asarray(a, dtype=np.float32).reshape(3)
+    return (ctypes.c_float * 3)(*[float(x) for x in a])
+
+
+def create_mesh_bbox_cube(box: AABox) -> AABox:
+    """createMeshBBCube (util.h:80-110): pad the box to a cube, then pad every
+    side by (max - min) / 10001."""
+    lib = _capi.load()
+    omin, omax = (ctypes.c_float * 3)(), (ctypes.c_float * 3)()
+    check(lib.vox_bbox_cube(_f3(box.min), _f3(box.max), omin, omax))
+    return AABox(np.array(omin[:], np.float32), np.array(omax[:], np.float32))
+
+
+createMeshBBCube = create_mesh_bbox_cube
+
+
+def table_bytes(grid: int) -> int:
+    """Bytes of a grid^3 bit table: ceil(G^3/32)*4 (main.cpp:190)."""
+    return int(_capi.load().vox_table_bytes(grid, grid, grid))
+
+
+def table_words(grid: int) -> int:
+    return table_bytes(grid) // 4
+
+
+def mesh_bbox(vertices: np.ndarray) -> AABox:
+    """Component-wise min/max over all vertices (trimesh2 ``need_bbox``, main.cpp:179)."""
+    v = np.asarray(vertices, dtype=np.float32).reshape(-1, 3)
+    return AABox(v.min(axis=0), v.max(axis=0))
+
+
+def voxinfo_for_mesh(vertices: np.ndarray, grid: int, n_triangles: int) -> VoxInfo:
+    """bbox -> createMeshBBCube -> voxinfo, as main.cpp:179-186 does."""
+    return VoxInfo(create_mesh_bbox_cube(mesh_bbox(vertices)), (grid, grid, grid), n_triangles)
+
+
+# ---- device entry points ------------------------------------------------------------
+def _torch():
+    import torch
+    return torch
+
+
+def _check_tensor(t, dtype_name: str, what: str):
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise TypeError(f"{what} must be a torch tensor on a HIP device")
+    if str(t.dtype) != dtype_name:
+        raise TypeError(f"{what} must be {dtype_name}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what} must be contiguous")
+
+
+def _stream_ptr(stream) -> int | None:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream) or None
+
+
+def _info_checked(info: VoxInfo, tris, table, grid_words_needed=True):
+    _check_tensor(tris, "torch.float32", "triangle_data")
+    if tris.numel() < 9 * info.n_triangles:
+        raise ValueError(f"triangle_data holds {tris.numel()} floats, need 9*{info.n_triangles}")
+    if table is not None:
+        if str(table.dtype) not in ("torch.int32", "torch.uint32"):
+            raise TypeError("vtable must be int32/uint32")
+        _check_tensor(table, str(table.dtype), "vtable")
+        if table.numel() * 4 < table_bytes(info.grid):
+            raise ValueError(f"vtable holds {table.numel()} words, need {table_words(info.grid)}")
+
+
+def voxelize_surface(info: VoxInfo, tris, morton: bool = False, table=None, accumulate: bool = False,
+                     algo: str = "tile", stream=None):
+    """Surface voxelization (stream-ordered).  Returns the int32 table tensor.
+
+    Without ``accumulate`` the table is fully defined by this call; with it
+    bits are OR-ed into the existing contents (reference semantics)."""
+    torch = _torch()
+    if table is None:
+        table = torch.empty(table_words(info.grid), dtype=torch.int32, device=tris.device)
+    _info_checked(info, tris, table)
+    flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | (VOX_FLAG_ALGO_DIRECT if algo == "direct" else 0)
+    lib = _capi.load()
+    check(lib.vox_surface(ctypes.byref(info._c), tris.data_ptr(), table.data_ptr(),
+                          VOX_LAYOUT_MORTON if morton else VOX_LAYOUT_LINEAR, flags, _stream_ptr(stream)))
+    return table
+
+
+def voxelize_solid_table(info: VoxInfo, tris, morton: bool = False, table=None, accumulate: bool = False,
+                         algo: str = "tile", stream=None, offrow=None):
+    """Solid (parity) voxelization (stream-ordered).  Returns the int32 table.
+
+    ``offrow``: optional int64 device tensor of one element to which the
+    number of column runs whose end left the row is added (the caller zeroes
+    it).  Such runs spill into the next rows (linear layout) or are cut at
+    the table end exactly as the reference indexes them; runs with a
+    negative end are empty."""
+    torch = _torch()
+    if table is None:
+        table = torch.empty(table_words(info.grid), dtype=torch.int32, device=tris.device)
+    _info_checked(info, tris, table)
+    flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | (VOX_FLAG_ALGO_DIRECT if algo == "direct" else 0)
+    cl_ptr = None
+    if offrow is not None:
+        _check_tensor(offrow, "torch.int64", "offrow")
+        cl_ptr = offrow.data_ptr()
+    lib = _capi.load()
+    check(lib.vox_solid(ctypes.byref(info._c), tris.data_ptr(), table.data_ptr(),
+                        VOX_LAYOUT_MORTON if morton else VOX_LAYOUT_LINEAR, flags, cl_ptr, _stream_ptr(stream)))
+    return table
+
+
+def voxelize(v: VoxInfo, triangle_data, vtable, use_thrust_path: bool = False, morton_code: bool = False):
+    """Drop-in for ``voxelize(info, triangle_data, vtable, useThrustPath, morton_code)``.
+
+    ORs the surface voxels into ``vtable`` (the caller zeroes it, as main.cpp:214/229
+    do) and returns after the device is done, like voxelize.cu:218.
+    ``use_thrust_path`` is accepted for source compatibility and has no effect."""
+    torch = _torch()
+    voxelize_surface(v, triangle_data, morton=morton_code, table=vtable, accumulate=True)
+    torch.cuda.synchronize(triangle_data.device)
+
+
+def voxelize_solid(v: VoxInfo, triangle_data, vtable, morton_code: bool = False):
+    """Drop-in for ``voxelize_solid(info, triangle_data, vtable, morton_code)``:
+    XORs the parity fill into ``vtable`` and synchronizes (voxelize_solid.cu:173)."""
+    torch = _torch()
+    voxelize_solid_table(v, triangle_data, morton=morton_code, table=vtable, accumulate=True)
+    torch.cuda.synchronize(triangle_data.device)

@@ -1,0 +1,93 @@
+"""ctypes binding of libvoxelizer_amd.so (include/cuda_voxelizer_amd/vox_capi.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (hipcc, gfx950).
+Loading fails loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvoxelizer_amd.so")
+
+VOX_OK = 0
+VOX_ERR_INVALID = 1
+VOX_ERR_HIP = 2
+VOX_ERR_OOM = 3
+VOX_ERR_UNSUPPORTED = 4
+VOX_ERR_NO_DEVICE = 5
+
+VOX_LAYOUT_LINEAR = 0
+VOX_LAYOUT_MORTON = 1
+
+VOX_FLAG_ACCUMULATE = 0x1
+VOX_FLAG_ALGO_DIRECT = 0x2
+
+_STATUS_NAMES = {0: "VOX_OK", 1: "VOX_ERR_INVALID", 2: "VOX_ERR_HIP", 3: "VOX_ERR_OOM",
+                 4: "VOX_ERR_UNSUPPORTED", 5: "VOX_ERR_NO_DEVICE"}
+
+
+class VoxError(RuntimeError):
+    """A C-ABI call returned a status other than VOX_OK."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{_STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class vox_info(ctypes.Structure):
+    """Byte layout of the reference ``voxinfo`` (util.h:50-61), 64 bytes."""
+    _fields_ = [
+        ("bbox_min", ctypes.c_float * 3),
+        ("bbox_max", ctypes.c_float * 3),
+        ("gridsize", ctypes.c_uint32 * 3),
+        ("_pad0", ctypes.c_uint32),
+        ("n_triangles", ctypes.c_uint64),
+        ("unit", ctypes.c_float * 3),
+        ("_pad1", ctypes.c_uint32),
+    ]
+
+
+assert ctypes.sizeof(vox_info) == 64
+
+_F3 = ctypes.c_float * 3
+_P = ctypes.c_void_p
+
+# name -> (restype, argtypes); every symbol declared in vox_capi.h
+SIGNATURES = {
+    "vox_init": (ctypes.c_int, [ctypes.c_int]),
+    "vox_last_error": (ctypes.c_char_p, []),
+    "vox_version": (ctypes.c_char_p, []),
+    "vox_table_bytes": (ctypes.c_size_t, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+    "vox_make_info": (ctypes.c_int, [_F3, _F3, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(vox_info)]),
+    "vox_bbox_cube": (ctypes.c_int, [_F3, _F3, _F3, _F3]),
+    "vox_surface": (ctypes.c_int, [ctypes.POINTER(vox_info), _P, _P, ctypes.c_int, ctypes.c_uint, _P]),
+    "vox_solid": (ctypes.c_int, [ctypes.POINTER(vox_info), _P, _P, ctypes.c_int, ctypes.c_uint, _P, _P]),
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load the HIP library (once) and bind every C-ABI symbol."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(hipcc, gfx950). There is no CPU fallback.")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != VOX_OK:
+        msg = load().vox_last_error()
+        raise VoxError(status, msg.decode() if msg else "")

@@ -1,0 +1,232 @@
+// capi.hip -- the C-ABI (vox_capi.h) and the C++ drop-in (voxelize.h).
+//
+// Host-side rules restated from the reference (Forceflow/cuda_voxelizer v0.6):
+//   voxinfo unit        util.h:56-61
+//   createMeshBBCube    util.h:80-110
+//   table size          main.cpp:190
+//   initCuda            util_cuda.cpp:4-42 (device probe; no exit on failure here)
+//   voxelize / voxelize_solid launchers voxelize.cu:192-238, voxelize_solid.cu:147-193
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "../../include/cuda_voxelizer_amd/vox_capi.h"
+#include "../../include/cuda_voxelizer_amd/voxelize.h"
+#include "vox_kernels.h"
+#include "vox_host.h"
+
+namespace voxamd {
+
+static thread_local std::string g_last_error;
+
+vox_status fail(vox_status s, const std::string& msg) {
+    g_last_error = msg;
+    return s;
+}
+
+vox_status hip_fail(hipError_t e, const char* what) {
+    return fail(VOX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Checks shared by every compute entry point.  The grid must be cubic (the
+// reference only builds cubic grids, main.cpp:186, and its CPU and GPU paths
+// index non-cubic tables differently, cpu_voxelizer.cpp:168 vs voxelize.cu:175);
+// the Morton layout needs a power-of-two edge below 2^16 so every code is
+// inside the table.
+vox_status validate(const vox_info* info, const float* tris, const uint32_t* table, int layout) {
+    if (!info) return fail(VOX_ERR_INVALID, "info is NULL");
+    const uint32_t G = info->gridsize[0];
+    if (G == 0 || info->gridsize[1] != G || info->gridsize[2] != G)
+        return fail(VOX_ERR_INVALID, "gridsize must be cubic and non-zero");
+    if (G > kMaxGrid) return fail(VOX_ERR_UNSUPPORTED, "grid edge above the supported maximum (8192)");
+    for (int k = 0; k < 3; k++) {
+        if (!(std::isfinite(info->unit[k]) && info->unit[k] > 0.0f))
+            return fail(VOX_ERR_INVALID, "unit must be finite and positive (degenerate bbox?)");
+        if (!std::isfinite(info->bbox_min[k])) return fail(VOX_ERR_INVALID, "bbox_min not finite");
+    }
+    if (layout != VOX_LAYOUT_LINEAR && layout != VOX_LAYOUT_MORTON)
+        return fail(VOX_ERR_INVALID, "layout must be VOX_LAYOUT_LINEAR or VOX_LAYOUT_MORTON");
+    if (layout == VOX_LAYOUT_MORTON && (G & (G - 1)) != 0)
+        return fail(VOX_ERR_UNSUPPORTED, "Morton layout needs a power-of-two grid");
+    if (info->n_triangles > 0 && !tris) return fail(VOX_ERR_INVALID, "triangle pointer is NULL");
+    if (!table) return fail(VOX_ERR_INVALID, "table pointer is NULL");
+    if (info->n_triangles > (uint64_t)kMaxTriangles)
+        return fail(VOX_ERR_UNSUPPORTED, "more than 2^31-1 triangles in one call");
+    return VOX_OK;
+}
+
+}  // namespace voxamd
+
+using namespace voxamd;
+
+extern "C" {
+
+const char* vox_last_error(void) { return g_last_error.c_str(); }
+
+const char* vox_version(void) { return "cuda_voxelizer_amd 0.1 (gfx950)"; }
+
+vox_status vox_init(int device) {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count < 1) return fail(VOX_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= count) return fail(VOX_ERR_INVALID, "device index out of range");
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    return VOX_OK;
+}
+
+size_t vox_table_bytes(uint32_t gx, uint32_t gy, uint32_t gz) {
+    const uint64_t voxels = (uint64_t)gx * gy * gz;
+    return (size_t)((voxels + 31) / 32) * 4;
+}
+
+vox_status vox_make_info(const float bbox_min[3], const float bbox_max[3], uint32_t grid, uint64_t n_triangles,
+                         vox_info* out) {
+    if (!bbox_min || !bbox_max || !out) return fail(VOX_ERR_INVALID, "NULL argument");
+    if (grid == 0) return fail(VOX_ERR_INVALID, "grid must be positive");
+    std::memset(out, 0, sizeof *out);
+    for (int k = 0; k < 3; k++) {
+        out->bbox_min[k] = bbox_min[k];
+        out->bbox_max[k] = bbox_max[k];
+        out->gridsize[k] = grid;
+        // unit.k = (bbox.max.k - bbox.min.k) / float(gridsize.k)
+        out->unit[k] = (bbox_max[k] - bbox_min[k]) / (float)grid;
+    }
+    out->n_triangles = n_triangles;
+    return VOX_OK;
+}
+
+vox_status vox_bbox_cube(const float in_min[3], const float in_max[3], float out_min[3], float out_max[3]) {
+    if (!in_min || !in_max || !out_min || !out_max) return fail(VOX_ERR_INVALID, "NULL argument");
+    float len[3], mn[3], mx[3];
+    for (int k = 0; k < 3; k++) {
+        len[k] = in_max[k] - in_min[k];
+        mn[k] = in_min[k];
+        mx[k] = in_max[k];
+    }
+    // std::max(a, b) == (a < b) ? b : a
+    const float m_yz = (len[1] < len[2]) ? len[2] : len[1];
+    const float max_length = (len[0] < m_yz) ? m_yz : len[0];
+    for (int k = 0; k < 3; k++) {
+        if (max_length != len[k]) {
+            const float half = (max_length - len[k]) / 2.0f;
+            mn[k] = in_min[k] - half;
+            mx[k] = in_max[k] + half;
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        const float eps = (mx[k] - mn[k]) / 10001.0f;
+        out_min[k] = mn[k] - eps;
+        out_max[k] = mx[k] + eps;
+    }
+    return VOX_OK;
+}
+
+vox_status vox_surface(const vox_info* info, const float* d_tris, uint32_t* d_table, int layout, unsigned flags,
+                       void* stream) {
+    vox_status s = validate(info, d_tris, d_table, layout);
+    if (s != VOX_OK) return s;
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t G = info->gridsize[0];
+    if (!(flags & VOX_FLAG_ACCUMULATE)) {
+        hipError_t e = hipMemsetAsync(d_table, 0, vox_table_bytes(G, G, G), st);
+        if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync(table)");
+    }
+    hipError_t e = launch_surface_direct(*info, d_tris, d_table, layout, st);
+    if (e != hipSuccess) return hip_fail(e, "surface kernel launch");
+    return VOX_OK;
+}
+
+vox_status vox_solid(const vox_info* info, const float* d_tris, uint32_t* d_table, int layout, unsigned flags,
+                     uint64_t* d_offrow, void* stream) {
+    vox_status s = validate(info, d_tris, d_table, layout);
+    if (s != VOX_OK) return s;
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t G = info->gridsize[0];
+    if (!(flags & VOX_FLAG_ACCUMULATE)) {
+        hipError_t e = hipMemsetAsync(d_table, 0, vox_table_bytes(G, G, G), st);
+        if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync(table)");
+    }
+    hipError_t e = launch_solid_direct(*info, d_tris, d_table, layout, (unsigned long long*)d_offrow, st);
+    if (e != hipSuccess) return hip_fail(e, "solid kernel launch");
+    return VOX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// C++ drop-in (voxelize.h)
+// ---------------------------------------------------------------------------
+static bool g_verbose = true;
+
+void voxelize_set_verbose(bool verbose) { g_verbose = verbose; }
+
+AABox<float3> createMeshBBCube(AABox<float3> box) {
+    const float in_min[3] = {box.min.x, box.min.y, box.min.z};
+    const float in_max[3] = {box.max.x, box.max.y, box.max.z};
+    float out_min[3], out_max[3];
+    vox_bbox_cube(in_min, in_max, out_min, out_max);
+    return AABox<float3>(make_float3(out_min[0], out_min[1], out_min[2]),
+                         make_float3(out_max[0], out_max[1], out_max[2]));
+}
+
+namespace {
+
+// checkCudaErrors-style: print and exit(EXIT_FAILURE) (helper_cuda.h:566-579).
+void die_on(vox_status s, const char* what) {
+    if (s == VOX_OK) return;
+    std::fprintf(stderr, "[Err] %s failed: %s\n", what, vox_last_error());
+    std::exit(EXIT_FAILURE);
+}
+
+void die_on_hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    std::fprintf(stderr, "[Err] %s failed: %s\n", what, hipGetErrorString(e));
+    std::exit(EXIT_FAILURE);
+}
+
+template <typename F>
+void run_timed(F&& f) {
+    hipEvent_t start, stop;
+    die_on_hip(hipEventCreate(&start), "hipEventCreate");
+    die_on_hip(hipEventCreate(&stop), "hipEventCreate");
+    die_on_hip(hipEventRecord(start, 0), "hipEventRecord");
+    f();
+    die_on_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    die_on_hip(hipEventRecord(stop, 0), "hipEventRecord");
+    die_on_hip(hipEventSynchronize(stop), "hipEventSynchronize");
+    float ms = 0.0f;
+    die_on_hip(hipEventElapsedTime(&ms, start, stop), "hipEventElapsedTime");
+    if (g_verbose) std::printf("[Perf] Voxelization GPU time: %.1f ms\n", ms);
+    die_on_hip(hipEventDestroy(start), "hipEventDestroy");
+    die_on_hip(hipEventDestroy(stop), "hipEventDestroy");
+}
+
+const vox_info* as_capi(const voxinfo& v) { return reinterpret_cast<const vox_info*>(&v); }
+
+}  // namespace
+
+void voxelize(const voxinfo& v, float* triangle_data, unsigned int* vtable, bool morton_code) {
+    run_timed([&] {
+        die_on(vox_surface(as_capi(v), triangle_data, vtable,
+                           morton_code ? VOX_LAYOUT_MORTON : VOX_LAYOUT_LINEAR, VOX_FLAG_ACCUMULATE, nullptr),
+               "voxelize");
+    });
+}
+
+void voxelize(const voxinfo& v, float* triangle_data, unsigned int* vtable, bool /*useThrustPath*/,
+              bool morton_code) {
+    voxelize(v, triangle_data, vtable, morton_code);
+}
+
+void voxelize_solid(const voxinfo& v, float* triangle_data, unsigned int* vtable, bool morton_code) {
+    run_timed([&] {
+        die_on(vox_solid(as_capi(v), triangle_data, vtable, morton_code ? VOX_LAYOUT_MORTON : VOX_LAYOUT_LINEAR,
+                         VOX_FLAG_ACCUMULATE, nullptr, nullptr),
+               "voxelize_solid");
+    });
+}

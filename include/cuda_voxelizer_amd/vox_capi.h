@@ -1,0 +1,104 @@
+/*
+ * vox_capi.h -- C-ABI of the MI355X mesh voxelizer (libvoxelizer_amd.so).
+ *
+ * Plain pointers and sizes only; every entry point returns a vox_status and
+ * never exits or throws.  Device pointers are HIP device (or managed)
+ * allocations; `stream` is a hipStream_t passed as void* (NULL = the default
+ * stream).  All compute entry points are stream-ordered and asynchronous with
+ * respect to the host unless stated otherwise.
+ *
+ * Reference interfaces replaced (Forceflow/cuda_voxelizer v0.6, src/):
+ *   vox_surface   <- void voxelize(const voxinfo&, float*, unsigned int*, bool)
+ *                    voxelize.cu:192-238 (kernel voxelize_triangle :58-190)
+ *   vox_solid     <- void voxelize_solid(const voxinfo&, float*, unsigned int*, bool)
+ *                    voxelize_solid.cu:147-193 (kernel :73-145)
+ *   vox_make_info <- voxinfo::voxinfo(AABox<float3>, uint3, size_t)  util.h:56-61
+ *   vox_bbox_cube <- createMeshBBCube<float3>(AABox<float3>)         util.h:80-110
+ *   vox_table_bytes <- vtable_size computation                       main.cpp:190
+ *   vox_init      <- initCuda()                                      util_cuda.cpp:4-42
+ */
+#ifndef CUDA_VOXELIZER_AMD_VOX_CAPI_H
+#define CUDA_VOXELIZER_AMD_VOX_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Byte-for-byte layout of the reference `voxinfo` (util.h:50-61): 64 bytes.
+   offset 0 bbox.min, 12 bbox.max, 24 gridsize, 40 n_triangles, 48 unit. */
+typedef struct vox_info {
+    float bbox_min[3];
+    float bbox_max[3];
+    uint32_t gridsize[3];
+    uint32_t _pad0;
+    uint64_t n_triangles;
+    float unit[3];
+    uint32_t _pad1;
+} vox_info;
+
+typedef enum vox_status {
+    VOX_OK = 0,
+    VOX_ERR_INVALID = 1,     /* bad argument: see vox_last_error() */
+    VOX_ERR_HIP = 2,         /* a HIP runtime call failed */
+    VOX_ERR_OOM = 3,         /* device workspace allocation failed */
+    VOX_ERR_UNSUPPORTED = 4, /* e.g. Morton layout with a non power-of-two grid */
+    VOX_ERR_NO_DEVICE = 5    /* no HIP device visible */
+} vox_status;
+
+/* Output layouts of the bit table (MSB-first bits, word = idx >> 5,
+   bit = 31 - (idx & 31); voxelize.cu:50-55). */
+#define VOX_LAYOUT_LINEAR 0 /* idx = x + y*G + z*G*G   (voxelize.cu:175-178) */
+#define VOX_LAYOUT_MORTON 1 /* idx = bit interleave x0 y0 z0 x1 ... (voxelize.cuh:20-34) */
+
+/* Flags for the compute entry points. */
+#define VOX_FLAG_ACCUMULATE 0x1u /* OR (surface) / XOR (solid) into the table's
+                                    current contents, like the reference; without
+                                    it the call defines every word of the table
+                                    (the zero-fill is fused into the kernels). */
+#define VOX_FLAG_ALGO_DIRECT 0x2u /* one-triangle-per-lane kernel with global
+                                    atomics (comparison path); default is the
+                                    binned tile kernels. */
+
+/* ---- setup / diagnostics ---------------------------------------------------- */
+vox_status vox_init(int device);            /* select + probe a device (initCuda) */
+const char* vox_last_error(void);           /* thread-local message of the last failure */
+const char* vox_version(void);
+
+/* ceil(gx*gy*gz / 32) * 4, computed exactly in integers (main.cpp:190 uses a
+   float division; identical for every grid whose voxel count is exact in fp32). */
+size_t vox_table_bytes(uint32_t gx, uint32_t gy, uint32_t gz);
+
+/* voxinfo constructor: unit = (max - min) / float(grid) per axis (util.h:56-61). */
+vox_status vox_make_info(const float bbox_min[3], const float bbox_max[3], uint32_t grid,
+                         uint64_t n_triangles, vox_info* out);
+
+/* createMeshBBCube (util.h:80-110): pad to a cube, then by (max-min)/10001. */
+vox_status vox_bbox_cube(const float in_min[3], const float in_max[3], float out_min[3],
+                         float out_max[3]);
+
+/* ---- hot path ------------------------------------------------------------------ */
+/* Surface voxelization of info->n_triangles AoS FP32 triangles d_tris[9*n]
+   (v0.xyz v1.xyz v2.xyz) into d_table (vox_table_bytes(G,G,G) bytes).
+   Bit-exact with the reference CPU path (cpu_voxelizer.cpp:35-183). */
+vox_status vox_surface(const vox_info* info, const float* d_tris, uint32_t* d_table, int layout,
+                       unsigned flags, void* stream);
+
+/* Solid (parity) voxelization; bit-exact with cpu_voxelizer.cpp:241-314.
+   A hit column (y,z) XORs the bits x = 0..xmax, xmax = int(xc/unit.x - 0.5).
+   When xmax leaves the row, the run follows the reference's indexing: for
+   xmax >= G a linear run continues into the following rows (bits past the
+   table end are dropped; a Morton run stops at G-1), and for xmax < 0 the
+   run is empty (the reference CPU path would wrap its unsigned xmax).
+   If d_offrow != NULL, the number of such runs is ADDED to *d_offrow
+   (uint64 device memory, stream-ordered; the caller zeroes it). */
+vox_status vox_solid(const vox_info* info, const float* d_tris, uint32_t* d_table, int layout,
+                     unsigned flags, uint64_t* d_offrow, void* stream);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* CUDA_VOXELIZER_AMD_VOX_CAPI_H */
