@@ -102,7 +102,7 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="torus512m", choices=sorted(CONFIGS))
-    ap.add_argument("--algo", default="tile", choices=["tile", "direct"])
+    ap.add_argument("--algo", default="auto", choices=["auto", "tile", "chunk", "direct"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -24,15 +24,23 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _capi
-from ._capi import (VOX_FLAG_ACCUMULATE, VOX_FLAG_ALGO_DIRECT, VOX_LAYOUT_LINEAR, VOX_LAYOUT_MORTON,
-                    VoxError, check, vox_info)
+from ._capi import (ALGO_FLAGS, VOX_FLAG_ACCUMULATE, VOX_FLAG_ALGO_CHUNK, VOX_FLAG_ALGO_DIRECT, VOX_FLAG_ALGO_TILE,
+                    VOX_LAYOUT_LINEAR, VOX_LAYOUT_MORTON, VoxError, check, vox_info)
 
 __all__ = [
     "AABox", "VoxInfo", "create_mesh_bbox_cube", "createMeshBBCube", "voxelize", "voxelize_solid",
     "voxelize_surface", "voxelize_solid_table", "table_bytes", "table_words", "mesh_bbox",
     "voxinfo_for_mesh", "VoxError", "VOX_LAYOUT_LINEAR", "VOX_LAYOUT_MORTON", "VOX_FLAG_ACCUMULATE",
-    "VOX_FLAG_ALGO_DIRECT",
+    "VOX_FLAG_ALGO_DIRECT", "VOX_FLAG_ALGO_TILE", "VOX_FLAG_ALGO_CHUNK", "ALGOS",
 ]
+
+ALGOS = tuple(ALGO_FLAGS)  # "auto", "direct", "tile", "chunk"
+
+
+def _algo_flag(algo: str) -> int:
+    if algo not in ALGO_FLAGS:
+        raise ValueError(f"algo must be one of {ALGOS}")
+    return ALGO_FLAGS[algo]
 
 
 @dataclass
@@ -176,7 +184,7 @@ def _info_checked(info: VoxInfo, tris, table, grid_words_needed=True):
 
 
 def voxelize_surface(info: VoxInfo, tris, morton: bool = False, table=None, accumulate: bool = False,
-                     algo: str = "tile", stream=None):
+                     algo: str = "auto", stream=None):
     """Surface voxelization (stream-ordered).  Returns the int32 table tensor.
 
     Without ``accumulate`` the table is fully defined by this call; with it
@@ -185,7 +193,7 @@ def voxelize_surface(info: VoxInfo, tris, morton: bool = False, table=None, accu
     if table is None:
         table = torch.empty(table_words(info.grid), dtype=torch.int32, device=tris.device)
     _info_checked(info, tris, table)
-    flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | (VOX_FLAG_ALGO_DIRECT if algo == "direct" else 0)
+    flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | _algo_flag(algo)
     lib = _capi.load()
     check(lib.vox_surface(ctypes.byref(info._c), tris.data_ptr(), table.data_ptr(),
                           VOX_LAYOUT_MORTON if morton else VOX_LAYOUT_LINEAR, flags, _stream_ptr(stream)))
@@ -193,7 +201,7 @@ def voxelize_surface(info: VoxInfo, tris, morton: bool = False, table=None, accu
 
 
 def voxelize_solid_table(info: VoxInfo, tris, morton: bool = False, table=None, accumulate: bool = False,
-                         algo: str = "tile", stream=None, offrow=None):
+                         algo: str = "auto", stream=None, offrow=None):
     """Solid (parity) voxelization (stream-ordered).  Returns the int32 table.
 
     ``offrow``: optional int64 device tensor of one element to which the
@@ -205,7 +213,7 @@ def voxelize_solid_table(info: VoxInfo, tris, morton: bool = False, table=None, 
     if table is None:
         table = torch.empty(table_words(info.grid), dtype=torch.int32, device=tris.device)
     _info_checked(info, tris, table)
-    flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | (VOX_FLAG_ALGO_DIRECT if algo == "direct" else 0)
+    flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | _algo_flag(algo)
     cl_ptr = None
     if offrow is not None:
         _check_tensor(offrow, "torch.int64", "offrow")

@@ -23,6 +23,9 @@ VOX_LAYOUT_MORTON = 1
 
 VOX_FLAG_ACCUMULATE = 0x1
 VOX_FLAG_ALGO_DIRECT = 0x2
+VOX_FLAG_ALGO_TILE = 0x4
+VOX_FLAG_ALGO_CHUNK = 0x8
+ALGO_FLAGS = {"auto": 0, "direct": VOX_FLAG_ALGO_DIRECT, "tile": VOX_FLAG_ALGO_TILE, "chunk": VOX_FLAG_ALGO_CHUNK}
 
 _STATUS_NAMES = {0: "VOX_OK", 1: "VOX_ERR_INVALID", 2: "VOX_ERR_HIP", 3: "VOX_ERR_OOM",
                  4: "VOX_ERR_UNSUPPORTED", 5: "VOX_ERR_NO_DEVICE"}
@@ -64,6 +67,7 @@ SIGNATURES = {
     "vox_bbox_cube": (ctypes.c_int, [_F3, _F3, _F3, _F3]),
     "vox_surface": (ctypes.c_int, [ctypes.POINTER(vox_info), _P, _P, ctypes.c_int, ctypes.c_uint, _P]),
     "vox_solid": (ctypes.c_int, [ctypes.POINTER(vox_info), _P, _P, ctypes.c_int, ctypes.c_uint, _P, _P]),
+    "vox_query_tile_plan": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]),
 }
 
 _lib = None

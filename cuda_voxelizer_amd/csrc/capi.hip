@@ -12,7 +12,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 
 #include "../../include/cuda_voxelizer_amd/vox_capi.h"
 #include "../../include/cuda_voxelizer_amd/voxelize.h"
@@ -56,6 +59,103 @@ vox_status validate(const vox_info* info, const float* tris, const uint32_t* tab
     if (!table) return fail(VOX_ERR_INVALID, "table pointer is NULL");
     if (info->n_triangles > (uint64_t)kMaxTriangles)
         return fail(VOX_ERR_UNSUPPORTED, "more than 2^31-1 triangles in one call");
+    return VOX_OK;
+}
+
+// ---- per (device, stream) workspace of the binned pipeline ----------------
+// The binned kernels keep the counter/count prefix of the workspace zeroed
+// between calls, so a cached workspace is only zero-filled when it is
+// (re)allocated.  Calls that share a stream are ordered by it; different
+// streams get different workspaces.
+struct Workspace {
+    void* ptr = nullptr;
+    uint64_t bytes = 0;
+    uint64_t tiles_cap = 0, n_cap = 0;
+};
+
+static std::mutex g_ws_mutex;
+static std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
+
+vox_status get_workspace(hipStream_t st, const TilePlan& P, uint64_t n, void** out, WorkspaceLayout* lay) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    Workspace& W = g_ws[std::make_pair(dev, st)];
+    if (W.ptr == nullptr || (uint64_t)P.ntiles > W.tiles_cap || n > W.n_cap) {
+        // counts are read in uint4 groups up to a multiple of 4096 by the scan
+        const uint64_t need = ((uint64_t)P.ntiles + 4095) & ~4095ull;
+        const uint64_t tiles_cap = W.tiles_cap > need ? W.tiles_cap : need;
+        uint64_t n_cap = W.n_cap > n ? W.n_cap : n;
+        if (n_cap < 4096) n_cap = 4096;
+        const WorkspaceLayout L = workspace_layout(tiles_cap, n_cap, P.max_tiles_per_tri);
+        if (W.ptr) {
+            e = hipStreamSynchronize(st);  // the old buffer may still be in use on this stream
+            if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+            (void)hipFree(W.ptr);
+            W.ptr = nullptr;
+        }
+        e = hipMalloc(&W.ptr, L.total);
+        if (e != hipSuccess) {
+            W = Workspace{};
+            return fail(VOX_ERR_OOM, std::string("workspace hipMalloc: ") + hipGetErrorString(e));
+        }
+        e = hipMemset(W.ptr, 0, L.total);
+        if (e != hipSuccess) return hip_fail(e, "hipMemset(workspace)");
+        W.bytes = L.total;
+        W.tiles_cap = tiles_cap;
+        W.n_cap = n_cap;
+    }
+    *out = W.ptr;
+    *lay = workspace_layout(W.tiles_cap, W.n_cap, P.max_tiles_per_tri);
+    return VOX_OK;
+}
+
+// One voxelization.  Algorithms: tile (binned, tile-owned writes; needs a
+// plan for this grid/layout), chunk (surface only, LDS word merging + global
+// atomics after a fill) and direct (one triangle per lane).  Without an
+// explicit flag: chunk for surface, tile for solid, direct when neither fits.
+vox_status run(const vox_info* info, const float* d_tris, uint32_t* d_table, int layout, unsigned flags,
+               uint64_t* d_offrow, void* stream, bool solid) {
+    vox_status s = validate(info, d_tris, d_table, layout);
+    if (s != VOX_OK) return s;
+    const unsigned algo_bits = flags & (VOX_FLAG_ALGO_DIRECT | VOX_FLAG_ALGO_TILE | VOX_FLAG_ALGO_CHUNK);
+    if (algo_bits & (algo_bits - 1)) return fail(VOX_ERR_INVALID, "at most one VOX_FLAG_ALGO_* flag");
+    if (solid && (flags & VOX_FLAG_ALGO_CHUNK)) return fail(VOX_ERR_INVALID, "the chunk algorithm is surface-only");
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t G = info->gridsize[0];
+    const uint64_t words = vox_table_bytes(G, G, G) / 4;
+    const bool accumulate = (flags & VOX_FLAG_ACCUMULATE) != 0;
+    TilePlan P;
+    const bool have_plan = make_plan((int)G, layout, solid, P);
+    enum { DIRECT, TILE, CHUNK } algo;
+    if (flags & VOX_FLAG_ALGO_DIRECT) algo = DIRECT;
+    else if (flags & VOX_FLAG_ALGO_TILE) algo = have_plan ? TILE : DIRECT;
+    else if (flags & VOX_FLAG_ALGO_CHUNK) algo = chunk_supported(G) ? CHUNK : DIRECT;
+    else if (solid) algo = have_plan ? TILE : DIRECT;
+    else algo = chunk_supported(G) ? CHUNK : (have_plan ? TILE : DIRECT);
+    hipError_t e;
+    if (algo == TILE) {
+        void* ws = nullptr;
+        WorkspaceLayout L;
+        s = get_workspace(st, P, info->n_triangles, &ws, &L);
+        if (s != VOX_OK) return s;
+        e = launch_tiled(*info, d_tris, d_table, P, solid, layout, accumulate, ws, L,
+                         (unsigned long long*)d_offrow, st);
+        if (e != hipSuccess) return hip_fail(e, "tile kernels launch");
+        return VOX_OK;
+    }
+    if (!accumulate) {
+        e = launch_fill_zero(d_table, words, st);
+        if (e != hipSuccess) return hip_fail(e, "table zero-fill");
+    }
+    if (algo == CHUNK)
+        e = launch_surface_chunk(*info, d_tris, d_table, layout, st);
+    else if (solid)
+        e = launch_solid_direct(*info, d_tris, d_table, layout, (unsigned long long*)d_offrow, st);
+    else
+        e = launch_surface_direct(*info, d_tris, d_table, layout, st);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return VOX_OK;
 }
 
@@ -126,34 +226,24 @@ vox_status vox_bbox_cube(const float in_min[3], const float in_max[3], float out
     return VOX_OK;
 }
 
+vox_status vox_query_tile_plan(uint32_t grid, int layout, int solid, int32_t out[8]) {
+    if (!out) return fail(VOX_ERR_INVALID, "NULL argument");
+    TilePlan P;
+    if (grid == 0 || grid > kMaxGrid || !make_plan((int)grid, layout, solid != 0, P))
+        return fail(VOX_ERR_UNSUPPORTED, "no tile plan for this grid/layout/mode");
+    out[0] = P.kind; out[1] = P.tx; out[2] = P.ty; out[3] = P.tz;
+    out[4] = P.ntiles; out[5] = P.tile_words; out[6] = P.row_words; out[7] = P.max_tiles_per_tri;
+    return VOX_OK;
+}
+
 vox_status vox_surface(const vox_info* info, const float* d_tris, uint32_t* d_table, int layout, unsigned flags,
                        void* stream) {
-    vox_status s = validate(info, d_tris, d_table, layout);
-    if (s != VOX_OK) return s;
-    hipStream_t st = (hipStream_t)stream;
-    const uint32_t G = info->gridsize[0];
-    if (!(flags & VOX_FLAG_ACCUMULATE)) {
-        hipError_t e = hipMemsetAsync(d_table, 0, vox_table_bytes(G, G, G), st);
-        if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync(table)");
-    }
-    hipError_t e = launch_surface_direct(*info, d_tris, d_table, layout, st);
-    if (e != hipSuccess) return hip_fail(e, "surface kernel launch");
-    return VOX_OK;
+    return run(info, d_tris, d_table, layout, flags, nullptr, stream, false);
 }
 
 vox_status vox_solid(const vox_info* info, const float* d_tris, uint32_t* d_table, int layout, unsigned flags,
                      uint64_t* d_offrow, void* stream) {
-    vox_status s = validate(info, d_tris, d_table, layout);
-    if (s != VOX_OK) return s;
-    hipStream_t st = (hipStream_t)stream;
-    const uint32_t G = info->gridsize[0];
-    if (!(flags & VOX_FLAG_ACCUMULATE)) {
-        hipError_t e = hipMemsetAsync(d_table, 0, vox_table_bytes(G, G, G), st);
-        if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync(table)");
-    }
-    hipError_t e = launch_solid_direct(*info, d_tris, d_table, layout, (unsigned long long*)d_offrow, st);
-    if (e != hipSuccess) return hip_fail(e, "solid kernel launch");
-    return VOX_OK;
+    return run(info, d_tris, d_table, layout, flags, d_offrow, stream, true);
 }
 
 }  // extern "C"

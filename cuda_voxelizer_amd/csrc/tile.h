@@ -1,0 +1,39 @@
+// tile.h -- output tiling shared by the binned (tile-owned) kernels.
+//
+// The bit table is split into tiles that are each a set of whole words; one
+// workgroup owns a tile, builds it in LDS and writes every one of its words
+// exactly once.  Tile shapes:
+//   linear layout (G % 32 == 0): all x (G bits per row) x ty rows in y x tz
+//     slabs in z; a tile is tz contiguous segments of ty*G/32 words.
+//   Morton layout, surface (G power of two): an aligned cube of edge T, whose
+//     T^3/32 words are one contiguous range; tile id = Morton code of the cube.
+//   Morton layout, solid: all x x T (y) x T (z); the parity fill needs whole
+//     x rows.  Output = G/T aligned cubes of T^3/32 contiguous words.
+#pragma once
+
+#include <stdint.h>
+
+namespace voxamd {
+
+enum TileKind : int { TILE_LINEAR = 0, TILE_MORTON_CUBE = 1, TILE_MORTON_ROWS = 2 };
+
+struct TilePlan {
+    int kind;
+    int G;            // grid edge
+    int tx, ty, tz;   // tile extent in voxels (tx == G for row tiles)
+    int ntx, nty, ntz;
+    int ntiles;
+    int tile_words;   // LDS words per tile
+    int row_words;    // G / 32 for row tiles
+    int max_tiles_per_tri;  // triangles spanning more tiles go to the wide path
+    int lg_cube;      // log2(tx) for Morton cubes
+    int pad;
+};
+
+// bytes of device workspace a call with this plan needs
+struct WorkspaceLayout {
+    uint64_t counts_off, offsets_off, list_off, wide_off, misc_off, spill_off, total;
+    uint64_t list_cap, wide_cap, spill_cap;
+};
+
+}  // namespace voxamd

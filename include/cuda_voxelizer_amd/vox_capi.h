@@ -58,9 +58,15 @@ typedef enum vox_status {
                                     current contents, like the reference; without
                                     it the call defines every word of the table
                                     (the zero-fill is fused into the kernels). */
-#define VOX_FLAG_ALGO_DIRECT 0x2u /* one-triangle-per-lane kernel with global
-                                    atomics (comparison path); default is the
-                                    binned tile kernels. */
+/* Algorithm selection (at most one; default: the library's choice per grid,
+   layout and mode -- chunk for surface, tile for solid).  All algorithms give
+   identical tables. */
+#define VOX_FLAG_ALGO_DIRECT 0x2u /* one triangle per lane, global atomics per word */
+#define VOX_FLAG_ALGO_TILE 0x4u   /* triangles binned to output tiles; one workgroup
+                                    builds each tile in LDS and writes it once */
+#define VOX_FLAG_ALGO_CHUNK 0x8u  /* surface only: one workgroup per chunk of
+                                    consecutive triangles merges words in an LDS
+                                    hash, then one atomicOr per distinct word */
 
 /* ---- setup / diagnostics ---------------------------------------------------- */
 vox_status vox_init(int device);            /* select + probe a device (initCuda) */
@@ -96,6 +102,15 @@ vox_status vox_surface(const vox_info* info, const float* d_tris, uint32_t* d_ta
    (uint64 device memory, stream-ordered; the caller zeroes it). */
 vox_status vox_solid(const vox_info* info, const float* d_tris, uint32_t* d_table, int layout,
                      unsigned flags, uint64_t* d_offrow, void* stream);
+
+/* ---- introspection ----------------------------------------------------------- */
+/* Output tiling the tile algorithm uses for a grid/layout/mode (no counterpart
+   in the reference).  out[0..7] = kind (0 linear rows, 1 Morton cubes, 2 Morton
+   rows), tile x/y/z extent in voxels, number of tiles, LDS words per tile,
+   words per row (row tiles), max tiles per binned triangle.  Returns
+   VOX_ERR_UNSUPPORTED when the tile algorithm does not handle the case (the
+   library then uses another algorithm). */
+vox_status vox_query_tile_plan(uint32_t grid, int layout, int solid, int32_t out[8]);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -92,3 +92,37 @@ def test_compute_entry_points_validate_before_touching_the_device(capi):
     assert capi.vox_surface(ctypes.byref(zero._c), dummy, dummy, 0, 0, None) == _capi.VOX_ERR_INVALID
     with pytest.raises(V.VoxError):
         V._capi.check(capi.vox_surface(None, dummy, dummy, 0, 0, None))
+    two = _capi.VOX_FLAG_ALGO_TILE | _capi.VOX_FLAG_ALGO_CHUNK
+    assert capi.vox_surface(ctypes.byref(info._c), dummy, dummy, 0, two, None) == _capi.VOX_ERR_INVALID
+    assert capi.vox_solid(ctypes.byref(info._c), dummy, dummy, 0, _capi.VOX_FLAG_ALGO_CHUNK, None,
+                          None) == _capi.VOX_ERR_INVALID
+
+
+def test_tile_plans_respect_kernel_bounds(capi):
+    """Every tile plan the library would use fits the kernels' assumptions:
+    tiles partition the grid, <= 65536 tiles (the scan), <= 64 KiB LDS per tile,
+    rows of <= 64 words (the suffix-XOR wave), Morton tiles are aligned cubes."""
+    out = (ctypes.c_int32 * 8)()
+    seen = 0
+    for g in list(range(1, 130)) + [192, 256, 384, 512, 640, 1024, 1536, 2048, 4096]:
+        for layout in (0, 1):
+            for solid in (0, 1):
+                st = capi.vox_query_tile_plan(g, layout, solid, out)
+                if st != 0:
+                    continue
+                seen += 1
+                kind, tx, ty, tz, ntiles, words, roww, kmax = list(out)
+                assert g % tx == 0 and g % ty == 0 and g % tz == 0
+                assert ntiles == (g // tx) * (g // ty) * (g // tz) <= 65536
+                assert 0 < words * 4 <= 65536
+                assert kmax >= 1
+                if kind == 0:
+                    assert layout == 0 and g % 32 == 0 and tx == g and roww == g // 32 <= 64
+                    assert words == roww * ty * tz
+                elif kind == 1:
+                    assert layout == 1 and not solid and tx == ty == tz and tx <= 32 and tx * tx * tx >= 64
+                    assert (g & (g - 1)) == 0 and words * 32 == tx ** 3
+                else:
+                    assert layout == 1 and solid and tx == g and ty == tz and ty ** 3 >= 32 and roww == g // 32 <= 64
+                    assert words == roww * ty * tz
+    assert seen > 20

@@ -14,7 +14,8 @@ from tests.conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
-ALGOS = ["direct"]
+ALGOS = ["auto", "tile", "chunk", "direct"]
+SOLID_ALGOS = ["auto", "tile", "direct"]
 
 
 def _dev_tris(tris, dev):
@@ -47,6 +48,8 @@ def _run(oracle, dev, tris, grid, morton, solid, algo, info=None):
         info = _info(tris.reshape(-1, 3), grid, tris.shape[0])
     d = _dev_tris(tris, dev)
     if solid:
+        if algo == "chunk":  # surface-only algorithm: the solid leg of a combined test uses the default
+            algo = "auto"
         offrow = torch.zeros(1, dtype=torch.int64, device=dev)
         t = V.voxelize_solid_table(info, d, morton=morton, algo=algo, offrow=offrow)
         exp, ocl, _ = oracle.solid(_oracle_info(oracle, info), tris, morton=morton)
@@ -103,7 +106,7 @@ def test_random_soup_surface(oracle, gpu, algo, grid):
         assert _mismatch(got, exp) == 0
 
 
-@pytest.mark.parametrize("algo", ALGOS)
+@pytest.mark.parametrize("algo", SOLID_ALGOS)
 @pytest.mark.parametrize("grid", [16, 33, 64])
 def test_closed_mesh_solid(oracle, gpu, algo, grid):
     from cuda_voxelizer_amd import meshgen
@@ -148,7 +151,8 @@ def test_empty_mesh_and_accumulate(oracle, gpu, algo):
     for solid in (False, True):
         t = torch.from_numpy(base.view(np.int32).copy()).to(gpu)
         if solid:
-            V.voxelize_solid_table(info, _dev_tris(tris, gpu), table=t, accumulate=True, algo=algo)
+            V.voxelize_solid_table(info, _dev_tris(tris, gpu), table=t, accumulate=True,
+                                   algo="auto" if algo == "chunk" else algo)
             exp, _, _ = oracle.solid(_oracle_info(oracle, info), tris, table=base.copy())
         else:
             V.voxelize_surface(info, _dev_tris(tris, gpu), table=t, accumulate=True, algo=algo)
@@ -180,7 +184,7 @@ def test_torus_slivers_512_morton_full(oracle, gpu, algo):
     tris = meshgen.torus_slivers()
     got, exp = _run(oracle, gpu, tris, 512, True, False, algo)
     assert _mismatch(got, exp) == 0
-    assert np.unpackbits(exp.view(np.uint8)).sum() > 1_000_000
+    assert np.unpackbits(exp.view(np.uint8)).sum() > 500_000
 
 
 @pytest.mark.parametrize("algo", ALGOS)
@@ -191,7 +195,7 @@ def test_torus_1024_linear_full(oracle, gpu, algo):
     assert _mismatch(got, exp) == 0
 
 
-@pytest.mark.parametrize("algo", ALGOS)
+@pytest.mark.parametrize("algo", SOLID_ALGOS)
 def test_sh_sphere_1024_solid_full(oracle, gpu, algo):
     from cuda_voxelizer_amd import meshgen
     v, f = meshgen.icosphere_sh()
