@@ -31,7 +31,7 @@ __all__ = [
     "AABox", "VoxInfo", "create_mesh_bbox_cube", "createMeshBBCube", "voxelize", "voxelize_solid",
     "voxelize_surface", "voxelize_solid_table", "table_bytes", "table_words", "mesh_bbox",
     "voxinfo_for_mesh", "VoxError", "VOX_LAYOUT_LINEAR", "VOX_LAYOUT_MORTON", "VOX_FLAG_ACCUMULATE",
-    "VOX_FLAG_ALGO_DIRECT", "VOX_FLAG_ALGO_TILE", "VOX_FLAG_ALGO_CHUNK", "ALGOS",
+    "VOX_FLAG_ALGO_DIRECT", "VOX_FLAG_ALGO_TILE", "VOX_FLAG_ALGO_CHUNK", "ALGOS", "BatchJob", "Batch",
 ]
 
 ALGOS = tuple(ALGO_FLAGS)  # "auto", "direct", "tile", "chunk"
@@ -241,3 +241,65 @@ def voxelize_solid(v: VoxInfo, triangle_data, vtable, morton_code: bool = False)
     torch = _torch()
     voxelize_solid_table(v, triangle_data, morton=morton_code, table=vtable, accumulate=True)
     torch.cuda.synchronize(triangle_data.device)
+
+
+# ---- batched mode -------------------------------------------------------------------
+@dataclass
+class BatchJob:
+    """One voxelization inside a :class:`Batch`."""
+    info: VoxInfo
+    tris: object                 # float32 device tensor, 9 * n_triangles
+    table: object                # int32 device tensor, table_words(G)
+    morton: bool = False
+    solid: bool = False
+    accumulate: bool = False
+    algo: str = "auto"
+    offrow: object = None        # solid: optional int64 device tensor of 1 element
+
+
+class Batch:
+    """Many voxelizations replayed as one HIP graph launch (``vox_batch_*``).
+
+    Pointers, voxinfo and flags are fixed when the batch is built; the
+    contents of the triangle and table tensors may change between launches.
+    The tensors must stay alive while the batch exists."""
+
+    def __init__(self, jobs: list[BatchJob], use_graph: bool = True):
+        lib = _capi.load()
+        self._jobs = list(jobs)  # keep the tensors alive
+        arr = (_capi.vox_batch_item * max(1, len(jobs)))()
+        for i, j in enumerate(jobs):
+            _info_checked(j.info, j.tris, j.table)
+            it = arr[i]
+            ctypes.memmove(ctypes.addressof(it.info), ctypes.addressof(j.info._c), 64)
+            it.d_tris = j.tris.data_ptr()
+            it.d_table = j.table.data_ptr()
+            it.d_offrow = j.offrow.data_ptr() if j.offrow is not None else None
+            it.layout = VOX_LAYOUT_MORTON if j.morton else VOX_LAYOUT_LINEAR
+            it.solid = 1 if j.solid else 0
+            it.flags = (VOX_FLAG_ACCUMULATE if j.accumulate else 0) | _algo_flag(j.algo)
+        handle = ctypes.c_void_p()
+        check(lib.vox_batch_create(arr, len(jobs), 0 if use_graph else _capi.VOX_BATCH_NO_GRAPH,
+                                   ctypes.byref(handle)))
+        self._h = handle
+
+    def launch(self, stream=None) -> None:
+        check(_capi.load().vox_batch_launch(self._h, _stream_ptr(stream)))
+
+    def info(self) -> dict:
+        out = (ctypes.c_int64 * 8)()
+        check(_capi.load().vox_batch_info(self._h, out))
+        keys = ("jobs", "triangles", "chunk_jobs", "tile_surface_jobs", "tile_solid_jobs", "direct_jobs",
+                "graph", "workspace_bytes")
+        return dict(zip(keys, [int(v) for v in out]))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _capi.load().vox_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -54,6 +54,22 @@ class vox_info(ctypes.Structure):
 
 assert ctypes.sizeof(vox_info) == 64
 
+
+class vox_batch_item(ctypes.Structure):
+    _fields_ = [
+        ("info", vox_info),
+        ("d_tris", ctypes.c_void_p),
+        ("d_table", ctypes.c_void_p),
+        ("d_offrow", ctypes.c_void_p),
+        ("layout", ctypes.c_int32),
+        ("solid", ctypes.c_int32),
+        ("flags", ctypes.c_uint32),
+        ("_pad", ctypes.c_uint32),
+    ]
+
+
+VOX_BATCH_NO_GRAPH = 0x1
+
 _F3 = ctypes.c_float * 3
 _P = ctypes.c_void_p
 
@@ -68,6 +84,14 @@ SIGNATURES = {
     "vox_surface": (ctypes.c_int, [ctypes.POINTER(vox_info), _P, _P, ctypes.c_int, ctypes.c_uint, _P]),
     "vox_solid": (ctypes.c_int, [ctypes.POINTER(vox_info), _P, _P, ctypes.c_int, ctypes.c_uint, _P, _P]),
     "vox_query_tile_plan": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]),
+    "vox_batch_create": (ctypes.c_int, [ctypes.POINTER(vox_batch_item), ctypes.c_int, ctypes.c_uint,
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "vox_batch_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "vox_batch_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "vox_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "vox_runner_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "vox_runner_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(vox_batch_item), ctypes.c_int]),
+    "vox_runner_destroy": (ctypes.c_int, [ctypes.c_void_p]),
 }
 
 _lib = None

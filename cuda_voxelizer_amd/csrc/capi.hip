@@ -124,7 +124,6 @@ vox_status run(const vox_info* info, const float* d_tris, uint32_t* d_table, int
     if (solid && (flags & VOX_FLAG_ALGO_CHUNK)) return fail(VOX_ERR_INVALID, "the chunk algorithm is surface-only");
     hipStream_t st = (hipStream_t)stream;
     const uint32_t G = info->gridsize[0];
-    const uint64_t words = vox_table_bytes(G, G, G) / 4;
     const bool accumulate = (flags & VOX_FLAG_ACCUMULATE) != 0;
     TilePlan P;
     const bool have_plan = make_plan((int)G, layout, solid, P);
@@ -146,7 +145,7 @@ vox_status run(const vox_info* info, const float* d_tris, uint32_t* d_table, int
         return VOX_OK;
     }
     if (!accumulate) {
-        e = launch_fill_zero(d_table, words, st);
+        e = launch_fill_zero(*info, d_table, st);
         if (e != hipSuccess) return hip_fail(e, "table zero-fill");
     }
     if (algo == CHUNK)

@@ -10,6 +10,7 @@
 // atomicOr per distinct word; words that do not fit in the hash table go
 // straight to global atomics.  The table is zero-filled beforehand (unless
 // accumulating).
+#include "jobs.h"
 #include "raster.h"
 #include "vox_kernels.h"
 
@@ -49,12 +50,20 @@ __device__ __forceinline__ void word_or(ChunkShared& S, uint32_t* __restrict__ t
     atomicOr(table + key, mask);  // hash table crowded: go to memory directly
 }
 
-__global__ void __launch_bounds__(kChunk) surface_chunk_kernel(vox_info info, const float* __restrict__ tris,
-                                                              uint32_t* __restrict__ table, int layout) {
+// Work unit: kChunk consecutive triangles of one job.
+__global__ void __launch_bounds__(kChunk) surface_chunk_kernel(ItemDev one, ItemMap M) {
     __shared__ ChunkShared S;
+    __shared__ BlockJob J;
+    resolve_job(one, M, blockIdx.x, J);
+    if (J.local == ~0u) return;
+    const vox_info info = J.it.info;
+    const float* __restrict__ tris = J.it.tris;
+    uint32_t* __restrict__ table = J.it.table;
+    const int layout = J.it.layout;
     const uint64_t n = info.n_triangles;
     const uint32_t G = info.gridsize[0];
-    const uint64_t first = (uint64_t)blockIdx.x * kChunk;
+    const uint64_t first = (uint64_t)J.local * kChunk;
+    if (first >= n) return;
     const int count = (int)((n - first) < (uint64_t)kChunk ? (n - first) : (uint64_t)kChunk);
     for (int i = threadIdx.x; i < kHashCap; i += kChunk) {
         S.keys[i] = kEmpty;
@@ -118,35 +127,70 @@ __global__ void __launch_bounds__(kChunk) surface_chunk_kernel(vox_info info, co
     }
 }
 
-// Zero-fill with 16-byte stores (a kernel, so it can live inside a captured graph).
-__global__ void __launch_bounds__(256) fill_zero_kernel(uint32_t* __restrict__ table, uint64_t words) {
-    const uint64_t n4 = words >> 2;
-    uint4* t4 = (uint4*)table;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n4; i += stride)
-        t4[i] = make_uint4(0u, 0u, 0u, 0u);
-    for (uint64_t i = (n4 << 2) + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < words; i += stride)
-        table[i] = 0u;
+// Zero-fill with 16-byte stores (a kernel, so it can live inside a captured
+// graph).  Work unit: kFillWords words of one job's table.
+constexpr uint64_t kFillWords = 16384;
+
+__global__ void __launch_bounds__(256) fill_zero_kernel(ItemDev one, ItemMap M) {
+    __shared__ BlockJob J;
+    resolve_job(one, M, blockIdx.x, J);
+    if (J.local == ~0u) return;
+    const uint32_t G = J.it.info.gridsize[0];
+    const uint64_t words = ((uint64_t)G * G * G + 31) / 32;
+    const uint64_t w0 = (uint64_t)J.local * kFillWords;
+    const uint64_t w1 = w0 + kFillWords < words ? w0 + kFillWords : words;
+    uint32_t* __restrict__ table = J.it.table;
+    if ((((uintptr_t)table) & 15u) == 0) {
+        uint4* t4 = (uint4*)table;
+        const uint64_t a = (w0 + 3) / 4, b = w1 / 4;  // whole uint4 inside [w0, w1)
+        for (uint64_t i = a + threadIdx.x; i < b; i += 256) t4[i] = make_uint4(0u, 0u, 0u, 0u);
+        for (uint64_t i = w0 + threadIdx.x; i < w1 && i < 4 * a; i += 256) table[i] = 0u;
+        for (uint64_t i = (b * 4 > w0 ? b * 4 : w0) + threadIdx.x; i < w1; i += 256) table[i] = 0u;
+    } else {
+        for (uint64_t i = w0 + threadIdx.x; i < w1; i += 256) table[i] = 0u;
+    }
 }
 
 bool chunk_supported(uint32_t G) { return (uint64_t)G * G * G / 32 < 0xFFFFFFFFull; }
 
-hipError_t launch_fill_zero(uint32_t* table, uint64_t words, hipStream_t stream) {
-    if (words == 0) return hipSuccess;
-    if (((uintptr_t)table & 15u) != 0) return hipMemsetAsync(table, 0, words * 4, stream);
-    uint64_t blocks = (words / 4 + 255) / 256;
-    if (blocks < 1) blocks = 1;
-    if (blocks > 8192) blocks = 8192;
-    fill_zero_kernel<<<(unsigned)blocks, 256, 0, stream>>>(table, words);
+uint32_t fill_units(uint32_t G) {
+    const uint64_t words = ((uint64_t)G * G * G + 31) / 32;
+    return (uint32_t)((words + kFillWords - 1) / kFillWords);
+}
+
+uint32_t chunk_units(uint64_t n_triangles) { return (uint32_t)((n_triangles + kChunk - 1) / kChunk); }
+
+hipError_t launch_fill_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, hipStream_t stream) {
+    if (units == 0) return hipSuccess;
+    fill_zero_kernel<<<units, 256, 0, stream>>>(one, M);
     return hipGetLastError();
+}
+
+hipError_t launch_chunk_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, hipStream_t stream) {
+    if (units == 0) return hipSuccess;
+    surface_chunk_kernel<<<units, kChunk, 0, stream>>>(one, M);
+    return hipGetLastError();
+}
+
+static ItemDev single_job(const vox_info& info, const float* tris, uint32_t* table, int layout) {
+    ItemDev one{};
+    one.info = info;
+    one.tris = tris;
+    one.table = table;
+    one.layout = layout;
+    return one;
+}
+
+hipError_t launch_fill_zero(const vox_info& info, uint32_t* table, hipStream_t stream) {
+    const ItemMap none{nullptr, nullptr, 1, 0};
+    return launch_fill_jobs(single_job(info, nullptr, table, 0), none, fill_units(info.gridsize[0]), stream);
 }
 
 hipError_t launch_surface_chunk(const vox_info& info, const float* tris, uint32_t* table, int layout,
                                 hipStream_t stream) {
     if (info.n_triangles == 0) return hipSuccess;
-    const uint64_t blocks = (info.n_triangles + kChunk - 1) / kChunk;
-    surface_chunk_kernel<<<(unsigned)blocks, kChunk, 0, stream>>>(info, tris, table, layout);
-    return hipGetLastError();
+    const ItemMap none{nullptr, nullptr, 1, 0};
+    return launch_chunk_jobs(single_job(info, tris, table, layout), none, chunk_units(info.n_triangles), stream);
 }
 
 }  // namespace voxamd

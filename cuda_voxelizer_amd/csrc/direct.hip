@@ -7,17 +7,26 @@
 // the arithmetic follows the CPU path bit for bit (vox_device.h), solid runs
 // XOR whole words instead of single bits, and bits of one word found by one
 // lane are merged before the atomic.
+#include "jobs.h"
 #include "vox_device.h"
 #include "vox_kernels.h"
 
 namespace voxamd {
 
-__global__ void __launch_bounds__(256) surface_direct_kernel(vox_info info, const float* __restrict__ tris,
-                                                             uint32_t* __restrict__ table, int layout) {
+// Work unit: 256 consecutive triangles of one job, one per lane.
+__global__ void __launch_bounds__(256) surface_direct_kernel(ItemDev one, ItemMap M) {
+    __shared__ BlockJob J;
+    resolve_job(one, M, blockIdx.x, J);
+    if (J.local == ~0u) return;
+    const vox_info info = J.it.info;
+    const float* __restrict__ tris = J.it.tris;
+    uint32_t* __restrict__ table = J.it.table;
+    const int layout = J.it.layout;
     const uint64_t n = info.n_triangles;
     const uint32_t G = info.gridsize[0];
-    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n;
-         t += (uint64_t)gridDim.x * blockDim.x) {
+    {
+        const uint64_t t = (uint64_t)J.local * 256 + threadIdx.x;
+        if (t >= n) return;
         float v[3][3];
         load_tri(tris, t, info, v);
         SurfTri T;
@@ -48,20 +57,27 @@ __global__ void __launch_bounds__(256) surface_direct_kernel(vox_info info, cons
     }
 }
 
-__global__ void __launch_bounds__(256) solid_direct_kernel(vox_info info, const float* __restrict__ tris,
-                                                           uint32_t* __restrict__ table, int layout,
-                                                           unsigned long long* __restrict__ offrow_out) {
+__global__ void __launch_bounds__(256) solid_direct_kernel(ItemDev one, ItemMap M) {
+    __shared__ BlockJob J;
+    resolve_job(one, M, blockIdx.x, J);
+    if (J.local == ~0u) return;
+    const vox_info info = J.it.info;
+    const float* __restrict__ tris = J.it.tris;
+    uint32_t* __restrict__ table = J.it.table;
+    const int layout = J.it.layout;
+    unsigned long long* __restrict__ offrow_out = J.it.offrow;
     const uint64_t n = info.n_triangles;
     const int G = (int)info.gridsize[0];
     const uint64_t total_bits = (uint64_t)G * (uint64_t)G * (uint64_t)G;
     unsigned long long offrow = 0;
-    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n;
-         t += (uint64_t)gridDim.x * blockDim.x) {
+    {
+        const uint64_t t = (uint64_t)J.local * 256 + threadIdx.x;
+        if (t >= n) return;
         float v[3][3];
         load_tri(tris, t, info, v);
         SolidTri S;
         solid_setup(v, info, S);
-        if (!S.valid) continue;
+        if (!S.valid) return;
         int ylo = (int)S.ymin, zlo = (int)S.zmin;
         if (ylo < 0) { offrow++; ylo = 0; }
         if (zlo < 0) { offrow++; zlo = 0; }
@@ -106,22 +122,41 @@ __global__ void __launch_bounds__(256) solid_direct_kernel(vox_info info, const 
     if (offrow_out && offrow) atomicAdd(offrow_out, offrow);
 }
 
+uint32_t direct_units(uint64_t n_triangles) { return (uint32_t)((n_triangles + 255) / 256); }
+
+hipError_t launch_direct_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, bool solid, hipStream_t stream) {
+    if (units == 0) return hipSuccess;
+    if (solid)
+        solid_direct_kernel<<<units, 256, 0, stream>>>(one, M);
+    else
+        surface_direct_kernel<<<units, 256, 0, stream>>>(one, M);
+    return hipGetLastError();
+}
+
 hipError_t launch_surface_direct(const vox_info& info, const float* tris, uint32_t* table, int layout,
                                  hipStream_t stream) {
     if (info.n_triangles == 0) return hipSuccess;
-    const uint64_t blocks = (info.n_triangles + 255) / 256;
-    const unsigned grid = (unsigned)(blocks < 65535ull * 64 ? blocks : 65535ull * 64);
-    surface_direct_kernel<<<grid, 256, 0, stream>>>(info, tris, table, layout);
-    return hipGetLastError();
+    ItemDev one{};
+    one.info = info;
+    one.tris = tris;
+    one.table = table;
+    one.layout = layout;
+    const ItemMap none{nullptr, nullptr, 1, 0};
+    return launch_direct_jobs(one, none, direct_units(info.n_triangles), false, stream);
 }
 
 hipError_t launch_solid_direct(const vox_info& info, const float* tris, uint32_t* table, int layout,
                                unsigned long long* offrow, hipStream_t stream) {
     if (info.n_triangles == 0) return hipSuccess;
-    const uint64_t blocks = (info.n_triangles + 255) / 256;
-    const unsigned grid = (unsigned)(blocks < 65535ull * 64 ? blocks : 65535ull * 64);
-    solid_direct_kernel<<<grid, 256, 0, stream>>>(info, tris, table, layout, offrow);
-    return hipGetLastError();
+    ItemDev one{};
+    one.info = info;
+    one.tris = tris;
+    one.table = table;
+    one.layout = layout;
+    one.offrow = offrow;
+    one.solid = 1;
+    const ItemMap none{nullptr, nullptr, 1, 0};
+    return launch_direct_jobs(one, none, direct_units(info.n_triangles), true, stream);
 }
 
 }  // namespace voxamd

@@ -13,6 +13,8 @@
 
 #include <stdint.h>
 
+#include "../../include/cuda_voxelizer_amd/vox_capi.h"
+
 namespace voxamd {
 
 enum TileKind : int { TILE_LINEAR = 0, TILE_MORTON_CUBE = 1, TILE_MORTON_ROWS = 2 };
@@ -27,6 +29,28 @@ struct TilePlan {
     int row_words;    // G / 32 for row tiles
     int max_tiles_per_tri;  // triangles spanning more tiles go to the wide path
     int lg_cube;      // log2(tx) for Morton cubes
+    int pad;
+};
+
+// One voxelization job as the kernels see it.
+struct ItemDev {
+    vox_info info;
+    const float* tris;
+    uint32_t* table;
+    unsigned long long* offrow;  // solid: out-of-row run counter (may be null)
+    TilePlan P;
+    int32_t layout, solid, accumulate;
+    uint32_t tile_base;  // first global tile id of this job (tile algorithm)
+    uint64_t tri_base;   // first global triangle id of this job
+};
+
+// Work mapping of one launch: a single job passed by value (items == nullptr)
+// or a device array of jobs with base[i] = first work unit (block, tile, ...)
+// of job i in this launch and base[nitems] = total units.
+struct ItemMap {
+    const ItemDev* items;
+    const uint32_t* base;
+    int nitems;
     int pad;
 };
 

@@ -23,6 +23,7 @@
 //
 // The per-voxel and per-column arithmetic is vox_device.h's, i.e. the
 // reference CPU path's (cpu_voxelizer.cpp:56-171, :255-310) bit for bit.
+#include "jobs.h"
 #include "raster.h"
 #include "tile.h"
 #include "vox_device.h"
@@ -142,115 +143,132 @@ __device__ __forceinline__ void staged_tri(const float* __restrict__ tris, uint6
 // ---------------------------------------------------------------------------
 // 1 + 3: binning
 // ---------------------------------------------------------------------------
+// Work unit: kBlock consecutive triangles of one job.  Tile ids are global
+// (job tile_base + local tile), list and wide entries are global triangle ids.
 template <bool SOLID, bool SCATTER>
-__global__ void __launch_bounds__(kBlock) bin_kernel(vox_info info, const float* __restrict__ tris, TilePlan P,
-                                                      unsigned* __restrict__ counts, unsigned* __restrict__ list,
-                                                      unsigned* __restrict__ misc, unsigned* __restrict__ wide_ids,
-                                                      unsigned long long* __restrict__ offrow) {
+__global__ void __launch_bounds__(kBlock) bin_kernel(ItemDev one, ItemMap M, unsigned* __restrict__ counts,
+                                                      unsigned* __restrict__ list, unsigned* __restrict__ misc,
+                                                      unsigned* __restrict__ wide_ids) {
     __shared__ float s_tri[kBlock * 9];
+    __shared__ BlockJob J;
+    resolve_job(one, M, blockIdx.x, J);
+    if (J.local == ~0u) return;
+    const vox_info info = J.it.info;
+    const TilePlan P = J.it.P;
     const uint64_t n = info.n_triangles;
+    const uint64_t base = (uint64_t)J.local * kBlock;
+    if (base >= n) return;
     unsigned long long events = 0;
-    for (uint64_t base = (uint64_t)blockIdx.x * kBlock; base < n; base += (uint64_t)gridDim.x * kBlock) {
-        const uint64_t t = base + threadIdx.x;
-        const bool active = t < n;
-        float v[3][3];
-        staged_tri(tris, base, n, s_tri, info, v);
-        Range R{0, -1, 0, -1, 0, -1};
-        if (active) {
-            if (SOLID) {
-                SolidTri S;
-                solid_setup(v, info, S);
-                int ylo, yhi, zlo, zhi;
-                if (S.valid && solid_columns(S, P.G, ylo, yhi, zlo, zhi, SCATTER ? nullptr : &events)) {
-                    R.a0 = 0; R.a1 = 0;
-                    R.b0 = ylo / P.ty; R.b1 = yhi / P.ty;
-                    R.c0 = zlo / P.tz; R.c1 = zhi / P.tz;
-                }
-            } else {
-                SurfTri T;
-                surface_setup(v, info, T);
-                R = surface_range(P, T);
+    const uint64_t t = base + threadIdx.x;
+    const bool active = t < n;
+    float v[3][3];
+    staged_tri(J.it.tris, base, n, s_tri, info, v);
+    Range R{0, -1, 0, -1, 0, -1};
+    if (active) {
+        if (SOLID) {
+            SolidTri S;
+            solid_setup(v, info, S);
+            int ylo, yhi, zlo, zhi;
+            if (S.valid && solid_columns(S, P.G, ylo, yhi, zlo, zhi, SCATTER ? nullptr : &events)) {
+                R.a0 = 0; R.a1 = 0;
+                R.b0 = ylo / P.ty; R.b1 = yhi / P.ty;
+                R.c0 = zlo / P.tz; R.c1 = zhi / P.tz;
             }
-        }
-        const int nt = (R.a1 < R.a0) ? 0 : R.count();
-        const bool wide = nt > P.max_tiles_per_tri;
-        if (!SCATTER && wide) wide_ids[atomicAdd(misc + MISC_WIDE, 1u)] = (unsigned)t;
-        for (int k = 0; k < P.max_tiles_per_tri; k++) {
-            const bool act = active && !wide && k < nt;
-            if (!__ballot(act)) break;
-            const int key = act ? range_tile(P, R, k) : 0;
-            const unsigned slot = run_add<SCATTER>(counts, key, act);
-            if (SCATTER && act) list[slot] = (unsigned)t;
+        } else {
+            SurfTri T;
+            surface_setup(v, info, T);
+            R = surface_range(P, T);
         }
     }
-    if (!SCATTER && SOLID && offrow && events) atomicAdd(offrow, events);
+    const unsigned gt = (unsigned)(J.it.tri_base + t);
+    const int nt = (R.a1 < R.a0) ? 0 : R.count();
+    const bool wide = nt > P.max_tiles_per_tri;
+    if (!SCATTER && wide) wide_ids[atomicAdd(misc + MISC_WIDE, 1u)] = gt;
+    unsigned* job_counts = counts + J.it.tile_base;
+    for (int k = 0; k < P.max_tiles_per_tri; k++) {
+        const bool act = active && !wide && k < nt;
+        if (!__ballot(act)) break;
+        const int key = act ? range_tile(P, R, k) : 0;
+        const unsigned slot = run_add<SCATTER>(job_counts, key, act);
+        if (SCATTER && act) list[slot] = gt;
+    }
+    if (!SCATTER && SOLID && J.it.offrow && events) atomicAdd(J.it.offrow, events);
 }
 
 // ---------------------------------------------------------------------------
 // 2: exclusive scan of counts -> offsets[0..ntiles], counts <- offsets (cursor)
 // ---------------------------------------------------------------------------
-// ntiles <= 65536 and the counts region is zero-padded to a multiple of 4096:
-// each of the 1024 threads loads up to 16 uint4 of consecutive counts in one
-// go, so the whole scan costs one memory latency.
-__global__ void __launch_bounds__(1024) bin_scan_kernel(TilePlan P, unsigned* __restrict__ counts,
+// Exclusive scan of counts[0 .. nt) into offsets (and the scatter cursor).
+// The counts region is zero-padded to a multiple of 4096: per pass each of the
+// 1024 threads loads up to 16 uint4 of consecutive counts at once (65536
+// counts per pass, one memory latency), passes chain through a carry.
+__global__ void __launch_bounds__(1024) bin_scan_kernel(unsigned nt, unsigned* __restrict__ counts,
                                                         unsigned* __restrict__ offsets, unsigned* __restrict__ misc) {
     __shared__ unsigned wave_sums[16];
+    __shared__ unsigned s_carry;
     const int tid = threadIdx.x;
     if (tid == 0) {
         misc[MISC_WIDE_N] = misc[MISC_WIDE];
         misc[MISC_WIDE] = 0u;
         misc[MISC_SPILL] = 0u;
         misc[MISC_SPILL_OVF] = 0u;
-    }
-    const int nt = P.ntiles;
-    const int per4 = (nt + 4095) / 4096;  // uint4 per thread, <= 16
-    const int beg = tid * per4 * 4;
-    uint4 v[16];
-    unsigned local = 0;
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-        if (q < per4) {
-            v[q] = *(const uint4*)(counts + beg + 4 * q);
-            local += v[q].x + v[q].y + v[q].z + v[q].w;
-        }
-    }
-    const int lane = tid & 63, wave = tid >> 6;
-    unsigned x = local;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const unsigned y = __shfl_up(x, d);
-        if (lane >= d) x += y;
-    }
-    if (lane == 63) wave_sums[wave] = x;
-    __syncthreads();
-    if (tid < 64) {
-        unsigned w = tid < 16 ? wave_sums[tid] : 0u;
-        unsigned e = w;
-#pragma unroll
-        for (int d = 1; d < 16; d <<= 1) {
-            const unsigned y = __shfl_up(e, d);
-            if (tid >= d) e += y;
-        }
-        if (tid < 16) wave_sums[tid] = e - w;
+        s_carry = 0u;
     }
     __syncthreads();
-    unsigned run = x - local + wave_sums[wave];
+    for (unsigned pass0 = 0; pass0 < (nt > 0 ? nt : 1u); pass0 += 65536u) {
+        const unsigned rem = nt - pass0 < 65536u ? nt - pass0 : 65536u;
+        const int per4 = (int)((rem + 4095u) / 4096u);  // uint4 per thread, <= 16
+        const unsigned beg = pass0 + (unsigned)tid * (unsigned)per4 * 4u;
+        uint4 v[16];
+        unsigned local = 0;
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-        if (q < per4) {
-            const unsigned c[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int i = beg + 4 * q + j;
-                if (i < nt) {
-                    offsets[i] = run;
-                    counts[i] = run;  // scatter cursor
-                }
-                run += c[j];
+        for (int q = 0; q < 16; q++) {
+            if (q < per4) {
+                v[q] = *(const uint4*)(counts + beg + 4 * q);
+                local += v[q].x + v[q].y + v[q].z + v[q].w;
             }
         }
+        const int lane = tid & 63, wave = tid >> 6;
+        unsigned x = local;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned y = __shfl_up(x, d);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) wave_sums[wave] = x;
+        __syncthreads();
+        if (tid < 64) {
+            const unsigned w = tid < 16 ? wave_sums[tid] : 0u;
+            unsigned e = w;
+#pragma unroll
+            for (int d = 1; d < 16; d <<= 1) {
+                const unsigned y = __shfl_up(e, d);
+                if (tid >= d) e += y;
+            }
+            if (tid < 16) wave_sums[tid] = e - w;
+        }
+        __syncthreads();
+        unsigned run = s_carry + x - local + wave_sums[wave];
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            if (q < per4) {
+                const unsigned c[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const unsigned i = beg + 4u * q + j;
+                    if (i < nt) {
+                        offsets[i] = run;
+                        counts[i] = run;  // scatter cursor
+                    }
+                    run += c[j];
+                }
+            }
+        }
+        __syncthreads();
+        if (tid == 1023) s_carry = run;
+        __syncthreads();
     }
-    if (tid == 1023) offsets[nt] = run;
+    if (tid == 0) offsets[nt] = s_carry;
 }
 
 // ---------------------------------------------------------------------------
@@ -351,26 +369,34 @@ __device__ __forceinline__ void flush_tile(const TilePlan& P, int tile, const Ti
     }
 }
 
-__global__ void __launch_bounds__(kBlock) raster_surface_kernel(vox_info info, const float* __restrict__ tris,
-                                                               TilePlan P, const unsigned* __restrict__ offsets,
+// Work unit: one tile of one job (global tile id = tile_base + local tile).
+__global__ void __launch_bounds__(kBlock) raster_surface_kernel(ItemDev one, ItemMap M,
+                                                               const unsigned* __restrict__ offsets,
                                                                unsigned* __restrict__ counts,
-                                                               const unsigned* __restrict__ list,
-                                                               uint32_t* __restrict__ table, int accumulate) {
+                                                               const unsigned* __restrict__ list) {
     extern __shared__ uint32_t lds[];
     __shared__ WaveRaster WR[kBlock / 64];
-    const int tile = blockIdx.x;
+    __shared__ BlockJob J;
+    resolve_job(one, M, blockIdx.x, J);
+    if (J.local == ~0u) return;
+    const vox_info info = J.it.info;
+    const TilePlan P = J.it.P;
+    const int tile = (int)J.local;
+    const unsigned gtile = J.it.tile_base + J.local;
+    const uint64_t tri_base = J.it.tri_base;
+    const float* __restrict__ tris = J.it.tris;
     const TileBox B = tile_box(P, tile);
     for (int i = threadIdx.x; i < P.tile_words; i += kBlock) lds[i] = 0u;
-    if (threadIdx.x == 0) counts[tile] = 0u;  // leave the workspace zeroed for the next call
+    if (threadIdx.x == 0) counts[gtile] = 0u;  // leave the workspace zeroed for the next call
     __syncthreads();
-    const unsigned beg = offsets[tile], end = offsets[tile + 1];
+    const unsigned beg = offsets[gtile], end = offsets[gtile + 1];
     WaveRaster& W = WR[threadIdx.x >> 6];
     for (unsigned base = beg; base < end; base += kBlock) {
         const unsigned i = base + threadIdx.x;
         const bool have = i < end;
         float v[3][3];
         if (have) {
-            load_tri(tris, list[i], info, v);
+            load_tri(tris, (uint64_t)list[i] - tri_base, info, v);
         } else {
 #pragma unroll
             for (int k = 0; k < 3; k++) v[k][0] = v[k][1] = v[k][2] = 0.0f;
@@ -395,7 +421,7 @@ __global__ void __launch_bounds__(kBlock) raster_surface_kernel(vox_info info, c
         }
     }
     __syncthreads();
-    flush_tile<false>(P, tile, B, lds, table, accumulate != 0);
+    flush_tile<false>(P, tile, B, lds, J.it.table, J.it.accumulate != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -477,28 +503,36 @@ __device__ __forceinline__ void flush_morton_rows(const TilePlan& P, int tile, c
     }
 }
 
-__global__ void __launch_bounds__(kBlock) raster_solid_kernel(vox_info info, const float* __restrict__ tris,
-                                                             TilePlan P, const unsigned* __restrict__ offsets,
+// Spill records: (first bit, last bit | job << 34) -- bit indices are < 2^33.
+constexpr int kSpillJobShift = 34;
+
+__global__ void __launch_bounds__(kBlock) raster_solid_kernel(ItemDev one, ItemMap M,
+                                                             const unsigned* __restrict__ offsets,
                                                              unsigned* __restrict__ counts,
                                                              const unsigned* __restrict__ list,
-                                                             uint32_t* __restrict__ table, int accumulate,
                                                              unsigned* __restrict__ misc,
                                                              unsigned long long* __restrict__ spill,
-                                                             unsigned spill_cap,
-                                                             unsigned long long* __restrict__ offrow) {
+                                                             unsigned spill_cap) {
     extern __shared__ uint32_t lds[];
-    const int tile = blockIdx.x;
+    __shared__ BlockJob J;
+    resolve_job(one, M, blockIdx.x, J);
+    if (J.local == ~0u) return;
+    const vox_info info = J.it.info;
+    const TilePlan P = J.it.P;
+    const int tile = (int)J.local;
+    const unsigned gtile = J.it.tile_base + J.local;
+    const uint64_t tri_base = J.it.tri_base;
+    const float* __restrict__ tris = J.it.tris;
     const TileBox B = tile_box(P, tile);
     const int G = P.G;
     for (int i = threadIdx.x; i < P.tile_words; i += kBlock) lds[i] = 0u;
-    if (threadIdx.x == 0) counts[tile] = 0u;
+    if (threadIdx.x == 0) counts[gtile] = 0u;
     __syncthreads();
     unsigned long long events = 0;
-    const unsigned beg = offsets[tile], end = offsets[tile + 1];
+    const unsigned beg = offsets[gtile], end = offsets[gtile + 1];
     for (unsigned i = beg + threadIdx.x; i < end; i += kBlock) {
-        const unsigned t = list[i];
         float v[3][3];
-        load_tri(tris, t, info, v);
+        load_tri(tris, (uint64_t)list[i] - tri_base, info, v);
         SolidTri S;
         solid_setup(v, info, S);
         int ylo, yhi, zlo, zhi;
@@ -529,7 +563,7 @@ __global__ void __launch_bounds__(kBlock) raster_solid_kernel(vox_info info, con
                             const unsigned slot = atomicAdd(misc + MISC_SPILL, 1u);
                             if (slot < spill_cap) {
                                 spill[2 * slot] = s0;
-                                spill[2 * slot + 1] = s1;
+                                spill[2 * slot + 1] = s1 | ((uint64_t)J.index << kSpillJobShift);
                             } else {
                                 misc[MISC_SPILL_OVF] = 1u;
                             }
@@ -541,36 +575,44 @@ __global__ void __launch_bounds__(kBlock) raster_solid_kernel(vox_info info, con
             }
         }
     }
-    if (offrow && events) atomicAdd(offrow, events);
+    if (J.it.offrow && events) atomicAdd(J.it.offrow, events);
     __syncthreads();
     suffix_xor_rows(P, lds);
     __syncthreads();
     if (P.kind == TILE_LINEAR)
-        flush_tile<true>(P, tile, B, lds, table, accumulate != 0);
+        flush_tile<true>(P, tile, B, lds, J.it.table, J.it.accumulate != 0);
     else
-        flush_morton_rows(P, tile, B, lds, table, accumulate != 0);
+        flush_morton_rows(P, tile, B, lds, J.it.table, J.it.accumulate != 0);
 }
 
 // ---------------------------------------------------------------------------
 // 5: wide triangles (and solid spills), global atomics, after the raster pass
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) wide_surface_kernel(vox_info info, const float* __restrict__ tris,
+// Wide triangles: global triangle ids, job found through M.base = per-job
+// triangle prefix.  One workgroup per wide triangle (persistent grid); its
+// (row, 32-voxel chunk) items are split across the workgroup.
+
+__global__ void __launch_bounds__(kBlock) wide_surface_kernel(ItemDev one, ItemMap M,
                                                              const unsigned* __restrict__ wide_ids,
-                                                             unsigned* __restrict__ misc,
-                                                             uint32_t* __restrict__ table, int layout) {
+                                                             const unsigned* __restrict__ misc) {
     const unsigned nw = misc[MISC_WIDE_N];
-    const uint32_t G = info.gridsize[0];
     for (unsigned w = blockIdx.x; w < nw; w += gridDim.x) {
+        const unsigned gt = wide_ids[w];
+        const ItemDev& it = job_ref(one, M, job_of_triangle(M, gt));
+        const vox_info info = it.info;
+        const uint32_t G = info.gridsize[0];
+        uint32_t* __restrict__ table = it.table;
+        const int layout = it.layout;
         float v[3][3];
-        load_tri(tris, wide_ids[w], info, v);
+        load_tri(it.tris, (uint64_t)gt - it.tri_base, info, v);
         SurfTri T;
         surface_setup(v, info, T);
         const int ny = T.hi[1] - T.lo[1] + 1, nz = T.hi[2] - T.lo[2] + 1;
         const int c0 = T.lo[0] >> 5, nch = (T.hi[0] >> 5) - c0 + 1;
         const uint64_t items = (uint64_t)ny * nz * nch;
-        for (uint64_t it = threadIdx.x; it < items; it += kBlock) {
-            const int ch = (int)(it % nch);
-            const uint64_t r = it / nch;
+        for (uint64_t k = threadIdx.x; k < items; k += kBlock) {
+            const int ch = (int)(k % nch);
+            const uint64_t r = k / nch;
             const int y = T.lo[1] + (int)(r % ny), z = T.lo[2] + (int)(r / ny);
             const float py = (float)y * info.unit[1], pz = (float)z * info.unit[2];
             if (!edges_pass(T.yz, py, pz)) continue;
@@ -626,27 +668,29 @@ __device__ __forceinline__ void xor_column_run(uint32_t* table, int layout, int 
     }
 }
 
-__global__ void __launch_bounds__(kBlock) wide_solid_kernel(vox_info info, const float* __restrict__ tris,
-                                                           TilePlan P, const unsigned* __restrict__ wide_ids,
-                                                           unsigned* __restrict__ misc,
+__global__ void __launch_bounds__(kBlock) wide_solid_kernel(ItemDev one, ItemMap M,
+                                                           const unsigned* __restrict__ wide_ids,
+                                                           const unsigned* __restrict__ misc,
                                                            const unsigned long long* __restrict__ spill,
-                                                           unsigned spill_cap, uint32_t* __restrict__ table,
-                                                           int layout, unsigned long long* __restrict__ offrow) {
+                                                           unsigned spill_cap) {
     const unsigned nw = misc[MISC_WIDE_N];
     const unsigned ns = misc[MISC_SPILL];
     const bool ovf = misc[MISC_SPILL_OVF] != 0u;
-    const int G = P.G;
-    unsigned long long events = 0;
     // (a) wide triangles: every hit column XORs its whole run with atomics
     for (unsigned w = blockIdx.x; w < nw; w += gridDim.x) {
+        const unsigned gt = wide_ids[w];
+        const ItemDev& it = job_ref(one, M, job_of_triangle(M, gt));
+        const vox_info info = it.info;
+        const int G = it.P.G;
         float v[3][3];
-        load_tri(tris, wide_ids[w], info, v);
+        load_tri(it.tris, (uint64_t)gt - it.tri_base, info, v);
         SolidTri S;
         solid_setup(v, info, S);
         int ylo, yhi, zlo, zhi;
         solid_columns(S, G, ylo, yhi, zlo, zhi, nullptr);
         const int ny = yhi - ylo + 1, nz = zhi - zlo + 1;
         const uint64_t cols = (ny > 0 && nz > 0) ? (uint64_t)ny * nz : 0;
+        unsigned long long events = 0;
         for (uint64_t c = threadIdx.x; c < cols; c += kBlock) {
             const int y = ylo + (int)(c / nz), z = zlo + (int)(c % nz);
             const float py = ((float)y + 0.5f) * info.unit[1], pz = ((float)z + 0.5f) * info.unit[2];
@@ -654,15 +698,18 @@ __global__ void __launch_bounds__(kBlock) wide_solid_kernel(vox_info info, const
             const int64_t xmax = column_xmax(S, info, py, pz);
             if (xmax < 0 || xmax >= G) events++;
             if (xmax < 0) continue;
-            xor_column_run(table, layout, G, y, z, xmax);
+            xor_column_run(it.table, it.layout, G, y, z, xmax);
         }
+        if (it.offrow && events) atomicAdd(it.offrow, events);
     }
     // (b) spills recorded by the raster pass (linear layout only); the words
     // of one (possibly long) spilled run are split across the workgroup
     if (!ovf) {
         const unsigned m = min(ns, spill_cap);
         for (unsigned s = blockIdx.x; s < m; s += gridDim.x) {
-            const uint64_t b0 = spill[2 * s], b1 = spill[2 * s + 1];
+            const uint64_t b0 = spill[2 * s];
+            const uint64_t b1 = spill[2 * s + 1] & ((1ull << kSpillJobShift) - 1ull);
+            uint32_t* __restrict__ table = job_ref(one, M, (int)(spill[2 * s + 1] >> kSpillJobShift)).table;
             for (uint64_t w = (b0 >> 5) + threadIdx.x; w <= (b1 >> 5); w += kBlock) {
                 const uint32_t lo = (w == (b0 >> 5)) ? (uint32_t)(b0 & 31u) : 0u;
                 const uint32_t hi = (w == (b1 >> 5)) ? (uint32_t)(b1 & 31u) : 31u;
@@ -671,10 +718,18 @@ __global__ void __launch_bounds__(kBlock) wide_solid_kernel(vox_info info, const
         }
     } else {
         // the spill list overflowed: recompute the spills of every binned triangle
-        const uint64_t n = info.n_triangles;
-        for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (uint64_t)gridDim.x * kBlock) {
+        const int nj = M.items ? M.nitems : 1;
+        const uint64_t total = M.items ? (uint64_t)M.base[M.nitems] : one.info.n_triangles;
+        for (uint64_t gt = (uint64_t)blockIdx.x * kBlock + threadIdx.x; gt < total;
+             gt += (uint64_t)gridDim.x * kBlock) {
+            const int j = nj > 1 ? job_of_triangle(M, (uint32_t)gt) : 0;
+            const ItemDev& it = job_ref(one, M, j);
+            if (it.layout != VOX_LAYOUT_LINEAR) continue;
+            const vox_info info = it.info;
+            const TilePlan P = it.P;
+            const int G = P.G;
             float v[3][3];
-            load_tri(tris, t, info, v);
+            load_tri(it.tris, gt - it.tri_base, info, v);
             SolidTri S;
             solid_setup(v, info, S);
             int ylo, yhi, zlo, zhi;
@@ -688,16 +743,15 @@ __global__ void __launch_bounds__(kBlock) wide_solid_kernel(vox_info info, const
                     if (!column_hit(S, py, pz)) continue;
                     const int64_t xmax = column_xmax(S, info, py, pz);
                     if (xmax < G) continue;
-                    const uint64_t total = (uint64_t)G * (uint64_t)G * (uint64_t)G;
+                    const uint64_t tot = (uint64_t)G * (uint64_t)G * (uint64_t)G;
                     const uint64_t row0 = ((uint64_t)y + (uint64_t)z * (uint64_t)G) * (uint64_t)G;
                     uint64_t s1 = row0 + (uint64_t)xmax;
-                    if (s1 > total - 1) s1 = total - 1;
-                    if (row0 + G <= s1) xor_linear_run(table, row0 + G, s1);
+                    if (s1 > tot - 1) s1 = tot - 1;
+                    if (row0 + G <= s1) xor_linear_run(it.table, row0 + G, s1);
                 }
             }
         }
     }
-    if (offrow && events) atomicAdd(offrow, events);
 }
 
 // ---------------------------------------------------------------------------
@@ -788,39 +842,57 @@ WorkspaceLayout workspace_layout(uint64_t tiles_cap, uint64_t n_cap, int max_til
     return L;
 }
 
+// Launch the tile pipeline for one job (items == nullptr) or a job array.
+// `M_bin`: base = per-job bin blocks; `M_tile`: base = per-job tiles;
+// `M_tri`: base = per-job triangle prefix.  total_* are the launch sizes.
+hipError_t launch_tiled_jobs(const ItemDev& one, const ItemMap& M_bin, const ItemMap& M_tile, const ItemMap& M_tri,
+                             uint32_t total_bin_blocks, uint32_t total_tiles, bool solid, uint32_t max_tile_words,
+                             void* ws, const WorkspaceLayout& L, hipStream_t stream) {
+    char* wb = (char*)ws;
+    unsigned* counts = (unsigned*)(wb + L.counts_off);
+    unsigned* offsets = (unsigned*)(wb + L.offsets_off);
+    unsigned* misc = (unsigned*)(wb + L.misc_off);
+    unsigned* list = (unsigned*)(wb + L.list_off);
+    unsigned* wide = (unsigned*)(wb + L.wide_off);
+    unsigned long long* spill = (unsigned long long*)(wb + L.spill_off);
+    const unsigned nb = total_bin_blocks > 0 ? total_bin_blocks : 1u;
+    const unsigned ntl = total_tiles > 0 ? total_tiles : 1u;
+    const size_t lds = (size_t)max_tile_words * 4;
+    if (solid) {
+        bin_kernel<true, false><<<nb, kBlock, 0, stream>>>(one, M_bin, counts, list, misc, wide);
+        bin_scan_kernel<<<1, 1024, 0, stream>>>(total_tiles, counts, offsets, misc);
+        bin_kernel<true, true><<<nb, kBlock, 0, stream>>>(one, M_bin, counts, list, misc, wide);
+        raster_solid_kernel<<<ntl, kBlock, lds, stream>>>(one, M_tile, offsets, counts, list, misc, spill,
+                                                          (unsigned)L.spill_cap);
+        wide_solid_kernel<<<kWideBlocks, kBlock, 0, stream>>>(one, M_tri, wide, misc, spill, (unsigned)L.spill_cap);
+    } else {
+        bin_kernel<false, false><<<nb, kBlock, 0, stream>>>(one, M_bin, counts, list, misc, wide);
+        bin_scan_kernel<<<1, 1024, 0, stream>>>(total_tiles, counts, offsets, misc);
+        bin_kernel<false, true><<<nb, kBlock, 0, stream>>>(one, M_bin, counts, list, misc, wide);
+        raster_surface_kernel<<<ntl, kBlock, lds, stream>>>(one, M_tile, offsets, counts, list);
+        wide_surface_kernel<<<kWideBlocks, kBlock, 0, stream>>>(one, M_tri, wide, misc);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_tiled(const vox_info& info, const float* tris, uint32_t* table, const TilePlan& P, bool solid,
                         int layout, bool accumulate, void* ws, const WorkspaceLayout& L,
                         unsigned long long* offrow, hipStream_t stream) {
-    char* base = (char*)ws;
-    unsigned* counts = (unsigned*)(base + L.counts_off);
-    unsigned* offsets = (unsigned*)(base + L.offsets_off);
-    unsigned* misc = (unsigned*)(base + L.misc_off);
-    unsigned* list = (unsigned*)(base + L.list_off);
-    unsigned* wide = (unsigned*)(base + L.wide_off);
-    unsigned long long* spill = (unsigned long long*)(base + L.spill_off);
-    const uint64_t n = info.n_triangles;
-    uint64_t blocks = (n + kBlock - 1) / kBlock;
-    if (blocks < 1) blocks = 1;
-    if (blocks > 8192) blocks = 8192;
-    const unsigned nb = (unsigned)blocks;
-    if (solid) {
-        bin_kernel<true, false><<<nb, kBlock, 0, stream>>>(info, tris, P, counts, list, misc, wide, offrow);
-        bin_scan_kernel<<<1, 1024, 0, stream>>>(P, counts, offsets, misc);
-        bin_kernel<true, true><<<nb, kBlock, 0, stream>>>(info, tris, P, counts, list, misc, wide, offrow);
-        raster_solid_kernel<<<P.ntiles, kBlock, (size_t)P.tile_words * 4, stream>>>(
-            info, tris, P, offsets, counts, list, table, accumulate ? 1 : 0, misc, spill, (unsigned)L.spill_cap,
-            offrow);
-        wide_solid_kernel<<<kWideBlocks, kBlock, 0, stream>>>(info, tris, P, wide, misc, spill,
-                                                              (unsigned)L.spill_cap, table, layout, offrow);
-    } else {
-        bin_kernel<false, false><<<nb, kBlock, 0, stream>>>(info, tris, P, counts, list, misc, wide, offrow);
-        bin_scan_kernel<<<1, 1024, 0, stream>>>(P, counts, offsets, misc);
-        bin_kernel<false, true><<<nb, kBlock, 0, stream>>>(info, tris, P, counts, list, misc, wide, offrow);
-        raster_surface_kernel<<<P.ntiles, kBlock, (size_t)P.tile_words * 4, stream>>>(
-            info, tris, P, offsets, counts, list, table, accumulate ? 1 : 0);
-        wide_surface_kernel<<<kWideBlocks, kBlock, 0, stream>>>(info, tris, wide, misc, table, layout);
-    }
-    return hipGetLastError();
+    ItemDev one{};
+    one.info = info;
+    one.tris = tris;
+    one.table = table;
+    one.offrow = offrow;
+    one.P = P;
+    one.layout = layout;
+    one.solid = solid ? 1 : 0;
+    one.accumulate = accumulate ? 1 : 0;
+    one.tile_base = 0;
+    one.tri_base = 0;
+    const ItemMap none{nullptr, nullptr, 1, 0};
+    const uint64_t blocks = (info.n_triangles + kBlock - 1) / kBlock;
+    return launch_tiled_jobs(one, none, none, none, (uint32_t)blocks, (uint32_t)P.ntiles, solid,
+                             (uint32_t)P.tile_words, ws, L, stream);
 }
 
 }  // namespace voxamd

@@ -14,14 +14,23 @@ hipError_t launch_surface_direct(const vox_info& info, const float* tris, uint32
 hipError_t launch_solid_direct(const vox_info& info, const float* tris, uint32_t* table, int layout,
                                unsigned long long* offrow, hipStream_t stream);
 
+uint32_t direct_units(uint64_t n_triangles);
+hipError_t launch_direct_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, bool solid, hipStream_t stream);
 bool make_plan(int G, int layout, bool solid, TilePlan& P);
 WorkspaceLayout workspace_layout(uint64_t tiles_cap, uint64_t n_cap, int max_tiles_per_tri);
+hipError_t launch_tiled_jobs(const ItemDev& one, const ItemMap& M_bin, const ItemMap& M_tile, const ItemMap& M_tri,
+                             uint32_t total_bin_blocks, uint32_t total_tiles, bool solid, uint32_t max_tile_words,
+                             void* ws, const WorkspaceLayout& L, hipStream_t stream);
 hipError_t launch_tiled(const vox_info& info, const float* tris, uint32_t* table, const TilePlan& P, bool solid,
                         int layout, bool accumulate, void* ws, const WorkspaceLayout& L,
                         unsigned long long* offrow, hipStream_t stream);
 
 bool chunk_supported(uint32_t G);
-hipError_t launch_fill_zero(uint32_t* table, uint64_t words, hipStream_t stream);
+hipError_t launch_fill_zero(const vox_info& info, uint32_t* table, hipStream_t stream);
+uint32_t fill_units(uint32_t G);
+uint32_t chunk_units(uint64_t n_triangles);
+hipError_t launch_fill_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, hipStream_t stream);
+hipError_t launch_chunk_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, hipStream_t stream);
 hipError_t launch_surface_chunk(const vox_info& info, const float* tris, uint32_t* table, int layout,
                                 hipStream_t stream);
 

@@ -103,6 +103,48 @@ vox_status vox_surface(const vox_info* info, const float* d_tris, uint32_t* d_ta
 vox_status vox_solid(const vox_info* info, const float* d_tris, uint32_t* d_table, int layout,
                      unsigned flags, uint64_t* d_offrow, void* stream);
 
+/* ---- batched mode (no counterpart in the reference) ---------------------------- */
+/* Many voxelizations -- meshes of any size, grids, layouts, surface or solid --
+   captured once into a HIP graph in which every kernel processes all jobs of
+   its algorithm class; vox_batch_launch is then one graph launch.  Pointers,
+   vox_info and flags are fixed at creation (a plan, as in FFT libraries);
+   triangle and table contents may change between launches.  Tables of the
+   items must not overlap. */
+typedef struct vox_batch_item {
+    vox_info info;         /* per-job voxinfo (cubic grid) */
+    const float* d_tris;   /* device AoS triangles, info.n_triangles of them */
+    uint32_t* d_table;     /* device table, vox_table_bytes(G,G,G) bytes */
+    uint64_t* d_offrow;    /* solid only: out-of-row run counter (added to), or NULL */
+    int32_t layout;        /* VOX_LAYOUT_LINEAR / VOX_LAYOUT_MORTON */
+    int32_t solid;         /* 0 surface, 1 solid */
+    uint32_t flags;        /* VOX_FLAG_ACCUMULATE, VOX_FLAG_ALGO_* */
+    uint32_t _pad;
+} vox_batch_item;
+
+typedef struct vox_batch vox_batch;
+
+#define VOX_BATCH_NO_GRAPH 0x1u /* launch the kernels directly each time (no graph) */
+
+vox_status vox_batch_create(const vox_batch_item* items, int count, unsigned flags, vox_batch** out);
+vox_status vox_batch_launch(vox_batch* batch, void* stream);
+/* out[0..7] = jobs, triangles, chunk jobs, tile surface jobs, tile solid jobs,
+   direct jobs, uses a graph, workspace bytes */
+vox_status vox_batch_info(const vox_batch* batch, int64_t out[8]);
+/* Waits for the device, then frees the batch. */
+vox_status vox_batch_destroy(vox_batch* batch);
+
+/* A re-fillable batch for serving: bound to one stream at creation; every
+   vox_runner_launch plans the given jobs, stages their descriptors through
+   pinned memory, copies them asynchronously and launches the batched kernels
+   on that stream without waiting for the device (it waits only when a
+   descriptor or workspace buffer has to grow).  Jobs of consecutive launches
+   on one runner run in stream order; use several runners (streams) to overlap
+   batches.  No graph: the job set changes every launch. */
+typedef struct vox_runner vox_runner;
+vox_status vox_runner_create(void* stream, vox_runner** out);
+vox_status vox_runner_launch(vox_runner* runner, const vox_batch_item* items, int count);
+vox_status vox_runner_destroy(vox_runner* runner);
+
 /* ---- introspection ----------------------------------------------------------- */
 /* Output tiling the tile algorithm uses for a grid/layout/mode (no counterpart
    in the reference).  out[0..7] = kind (0 linear rows, 1 Morton cubes, 2 Morton
