@@ -207,6 +207,14 @@ def run_batch(args) -> dict:
     meshes = meshgen.batch_meshes(256)
     jobs = _batch_jobs(meshes, dev)
     stream = torch.cuda.current_stream()
+    from cuda_voxelizer_amd.autotune import Tuner
+    t_tune = time.perf_counter()
+    tuner = Tuner()
+    tuner.tune_jobs(jobs)  # per-class algorithm choice, cached on disk per device
+    t_tune = time.perf_counter() - t_tune
+    algo_mix = {}
+    for j in jobs:
+        algo_mix[j.algo] = algo_mix.get(j.algo, 0) + 1
     batch = V.Batch(jobs, use_graph=True)
     steps = max(1, min(args.steps, 10))
 
@@ -243,7 +251,7 @@ def run_batch(args) -> dict:
                    "n_meshes": len(meshes), "n_triangles": ntri},
         "triangles_per_s": ntri / t_batch,
         "plain_meshes_per_s": len(meshes) / t_plain, "speedup_vs_plain": t_plain / t_batch,
-        "batch_equals_plain": same, "batch_info": batch.info(),
+        "batch_equals_plain": same, "batch_info": batch.info(), "autotune_s": t_tune, "algo_mix": algo_mix,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": alg_bytes,
                      "launch_ms": t_batch * 1e3},

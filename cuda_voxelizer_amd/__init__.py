@@ -32,6 +32,8 @@ __all__ = [
     "voxelize_surface", "voxelize_solid_table", "table_bytes", "table_words", "mesh_bbox",
     "voxinfo_for_mesh", "VoxError", "VOX_LAYOUT_LINEAR", "VOX_LAYOUT_MORTON", "VOX_FLAG_ACCUMULATE",
     "VOX_FLAG_ALGO_DIRECT", "VOX_FLAG_ALGO_TILE", "VOX_FLAG_ALGO_CHUNK", "ALGOS", "BatchJob", "Batch",
+    "mesh_bbox_device", "pack_triangles", "upload", "convert_layout", "count_voxels", "check_voxel",
+    "prepare_mesh",
 ]
 
 ALGOS = tuple(ALGO_FLAGS)  # "auto", "direct", "tile", "chunk"
@@ -303,3 +305,74 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ---- mesh preparation and table utilities -----------------------------------------------
+def mesh_bbox_device(vertices, stream=None) -> AABox:
+    """Device min/max of a float32 [V*3] vertex tensor (trimesh2 need_bbox)."""
+    torch = _torch()
+    _check_tensor(vertices, "torch.float32", "vertices")
+    out = torch.empty(6, dtype=torch.float32, device=vertices.device)
+    check(_capi.load().vox_mesh_bbox(vertices.data_ptr(), vertices.numel() // 3, out.data_ptr(),
+                                     _stream_ptr(stream)))
+    h = out.cpu().numpy()
+    return AABox(h[:3], h[3:])
+
+
+def pack_triangles(vertices, faces, stream=None):
+    """Indexed mesh (float32 [V*3], int32 [F*3] device tensors) -> AoS float32
+    [F*9] triangles (meshToGPU_managed).  Raises if a face index is out of range."""
+    torch = _torch()
+    _check_tensor(vertices, "torch.float32", "vertices")
+    _check_tensor(faces, "torch.int32", "faces")
+    nf = faces.numel() // 3
+    out = torch.empty(nf * 9, dtype=torch.float32, device=vertices.device)
+    bad = torch.zeros(1, dtype=torch.int64, device=vertices.device)
+    check(_capi.load().vox_pack_triangles(vertices.data_ptr(), vertices.numel() // 3, faces.data_ptr(), nf,
+                                          out.data_ptr(), bad.data_ptr(), _stream_ptr(stream)))
+    if int(bad.item()):
+        raise ValueError(f"{int(bad.item())} face indices out of range")
+    return out
+
+
+def upload(host_array: np.ndarray, device="cuda:0", stream=None):
+    """Host array -> new device tensor through pinned double-buffered staging."""
+    torch = _torch()
+    a = np.ascontiguousarray(host_array)
+    out = torch.empty(a.shape, dtype=getattr(torch, str(a.dtype)), device=device)
+    check(_capi.load().vox_upload_async(a.ctypes.data, a.nbytes, out.data_ptr(), _stream_ptr(stream)))
+    torch.cuda.current_stream().synchronize() if stream is None else stream.synchronize()
+    return out
+
+
+def prepare_mesh(vertices: np.ndarray, faces: np.ndarray, grid: int, device="cuda:0"):
+    """main.cpp:174-190 on the device: upload, bbox, createMeshBBCube, voxinfo,
+    AoS packing.  Returns (VoxInfo, triangle tensor)."""
+    v = upload(np.asarray(vertices, np.float32).reshape(-1), device)
+    f = upload(np.asarray(faces, np.int32).reshape(-1), device)
+    box = mesh_bbox_device(v)
+    info = VoxInfo(create_mesh_bbox_cube(box), (grid, grid, grid), f.numel() // 3)
+    return info, pack_triangles(v, f)
+
+
+def convert_layout(table, grid: int, to_morton: bool, stream=None):
+    """Linear <-> Morton re-layout of a power-of-two grid table (new tensor)."""
+    torch = _torch()
+    out = torch.empty_like(table)
+    check(_capi.load().vox_convert_layout(table.data_ptr(), out.data_ptr(), grid, int(to_morton),
+                                          _stream_ptr(stream)))
+    return out
+
+
+def count_voxels(table, stream=None) -> int:
+    torch = _torch()
+    c = torch.zeros(1, dtype=torch.int64, device=table.device)
+    check(_capi.load().vox_count_voxels(table.data_ptr(), table.numel(), c.data_ptr(), _stream_ptr(stream)))
+    return int(c.item())
+
+
+def check_voxel(host_table: np.ndarray, grid: int, x: int, y: int, z: int, morton: bool = False) -> bool:
+    """checkVoxel (util.h:25-38) on a host table (either layout)."""
+    t = np.ascontiguousarray(host_table, dtype=np.uint32)
+    return bool(_capi.load().vox_check_voxel(t.ctypes.data, grid, x, y, z,
+                                            VOX_LAYOUT_MORTON if morton else VOX_LAYOUT_LINEAR))

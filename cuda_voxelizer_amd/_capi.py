@@ -89,6 +89,13 @@ SIGNATURES = {
     "vox_batch_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "vox_batch_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "vox_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "vox_mesh_bbox": (ctypes.c_int, [_P, ctypes.c_uint64, _P, _P]),
+    "vox_pack_triangles": (ctypes.c_int, [_P, ctypes.c_uint64, _P, ctypes.c_uint64, _P, _P, _P]),
+    "vox_upload_async": (ctypes.c_int, [_P, ctypes.c_size_t, _P, _P]),
+    "vox_convert_layout": (ctypes.c_int, [_P, _P, ctypes.c_uint32, ctypes.c_int, _P]),
+    "vox_count_voxels": (ctypes.c_int, [_P, ctypes.c_uint64, _P, _P]),
+    "vox_check_voxel": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.c_int]),
     "vox_runner_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "vox_runner_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(vox_batch_item), ctypes.c_int]),
     "vox_runner_destroy": (ctypes.c_int, [ctypes.c_void_p]),

@@ -1,0 +1,224 @@
+// prep.hip -- the steps either side of the hot path, on the device.
+//
+//   vox_mesh_bbox        component-wise min/max of the vertices (trimesh2's
+//                        need_bbox, main.cpp:179) as a device reduction
+//   vox_pack_triangles   indexed mesh -> AoS FP32 triangles (main.cpp:61-80,
+//                        meshToGPU_managed) as a gather kernel, with index checks
+//   vox_upload_async     host -> device through double-buffered pinned staging
+//                        (replaces the reference's managed-memory upload)
+//   vox_convert_layout   linear <-> Morton bit tables (voxelize.cuh:20-34 vs
+//                        util.h:25-38 / checkVoxel)
+//   vox_count_voxels     number of set voxels (popcount reduction)
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/cuda_voxelizer_amd/vox_capi.h"
+#include "vox_device.h"
+#include "vox_host.h"
+
+namespace voxamd {
+
+// Order-preserving map of a finite float to uint32 (negative values reversed).
+__device__ __forceinline__ uint32_t float_key(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_float(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+__global__ void __launch_bounds__(256) bbox_init_kernel(uint32_t* keys) {
+    if (threadIdx.x < 3) keys[threadIdx.x] = 0xFFFFFFFFu;       // min
+    if (threadIdx.x >= 3 && threadIdx.x < 6) keys[threadIdx.x] = 0u;  // max
+}
+
+__global__ void __launch_bounds__(256) bbox_kernel(const float* __restrict__ v, uint64_t n, uint32_t* keys) {
+    uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const uint32_t key = float_key(v[3 * i + k]);
+            mn[k] = key < mn[k] ? key : mn[k];
+            mx[k] = key > mx[k] ? key : mx[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        for (int d = 32; d > 0; d >>= 1) {
+            const uint32_t a = __shfl_xor(mn[k], d), b = __shfl_xor(mx[k], d);
+            mn[k] = a < mn[k] ? a : mn[k];
+            mx[k] = b > mx[k] ? b : mx[k];
+        }
+    }
+    if (__lane_id() == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            atomicMin(keys + k, mn[k]);
+            atomicMax(keys + 3 + k, mx[k]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64) bbox_finish_kernel(const uint32_t* keys, float* out) {
+    if (threadIdx.x < 6) out[threadIdx.x] = key_float(keys[threadIdx.x]);
+}
+
+__global__ void __launch_bounds__(256) pack_kernel(const float* __restrict__ v, uint64_t nv,
+                                                   const int32_t* __restrict__ faces, uint64_t nf,
+                                                   float* __restrict__ out, unsigned long long* bad) {
+    unsigned long long nbad = 0;
+    for (uint64_t f = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; f < nf; f += (uint64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int32_t idx = faces[3 * f + c];
+            const bool ok = idx >= 0 && (uint64_t)idx < nv;
+            if (!ok) nbad++;
+#pragma unroll
+            for (int k = 0; k < 3; k++) out[9 * f + 3 * c + k] = ok ? v[3 * (uint64_t)idx + k] : 0.0f;
+        }
+    }
+    if (bad && nbad) atomicAdd(bad, nbad);
+}
+
+// Linear <-> Morton: one thread per destination word gathers its 32 voxels.
+__global__ void __launch_bounds__(256) convert_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                                      uint32_t G, int to_morton) {
+    const uint64_t words = ((uint64_t)G * G * G) >> 5;
+    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < words;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t out = 0;
+        for (int j = 0; j < 32; j++) {
+            const uint64_t di = (w << 5) | (uint64_t)j;  // destination voxel index
+            uint32_t x, y, z;
+            if (to_morton) {  // di is a Morton code
+                x = y = z = 0;
+                for (int b = 0; b < 16; b++) {
+                    x |= (uint32_t)((di >> (3 * b)) & 1u) << b;
+                    y |= (uint32_t)((di >> (3 * b + 1)) & 1u) << b;
+                    z |= (uint32_t)((di >> (3 * b + 2)) & 1u) << b;
+                }
+            } else {
+                x = (uint32_t)(di % G);
+                y = (uint32_t)((di / G) % G);
+                z = (uint32_t)(di / ((uint64_t)G * G));
+            }
+            const uint64_t si = to_morton ? (uint64_t)x + (uint64_t)y * G + (uint64_t)z * G * G : morton3(x, y, z);
+            if ((src[si >> 5] >> (31u - (uint32_t)(si & 31u))) & 1u) out |= 1u << (31 - j);
+        }
+        dst[w] = out;
+    }
+}
+
+__global__ void __launch_bounds__(256) count_kernel(const uint32_t* __restrict__ t, uint64_t words,
+                                                    unsigned long long* out) {
+    unsigned long long c = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
+        c += (unsigned long long)__popc(t[i]);
+    for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d);
+    if (__lane_id() == 0 && c) atomicAdd(out, c);
+}
+
+static unsigned grid_for(uint64_t n) {
+    uint64_t b = (n + 255) / 256;
+    if (b < 1) b = 1;
+    if (b > 4096) b = 4096;
+    return (unsigned)b;
+}
+
+// pinned staging for vox_upload_async (double buffered, per process)
+static std::mutex g_up_mutex;
+static void* g_up_buf[2] = {nullptr, nullptr};
+static hipEvent_t g_up_ev[2] = {nullptr, nullptr};
+constexpr size_t kUpChunk = 8u << 20;
+
+}  // namespace voxamd
+
+using namespace voxamd;
+
+extern "C" {
+
+vox_status vox_mesh_bbox(const float* d_vertices, uint64_t n_vertices, float* d_out6, void* stream) {
+    if (!d_out6 || (n_vertices && !d_vertices)) return fail(VOX_ERR_INVALID, "NULL argument");
+    if (n_vertices == 0) return fail(VOX_ERR_INVALID, "empty vertex array");
+    hipStream_t st = (hipStream_t)stream;
+    // the 6 keys live in the output buffer itself until the finish kernel converts them
+    uint32_t* keys = (uint32_t*)d_out6;
+    bbox_init_kernel<<<1, 64, 0, st>>>(keys);
+    bbox_kernel<<<grid_for(n_vertices), 256, 0, st>>>(d_vertices, n_vertices, keys);
+    bbox_finish_kernel<<<1, 64, 0, st>>>(keys, d_out6);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? VOX_OK : hip_fail(e, "bbox kernels");
+}
+
+vox_status vox_pack_triangles(const float* d_vertices, uint64_t n_vertices, const int32_t* d_faces,
+                              uint64_t n_faces, float* d_tris, uint64_t* d_bad, void* stream) {
+    if (n_faces && (!d_vertices || !d_faces || !d_tris)) return fail(VOX_ERR_INVALID, "NULL argument");
+    if (n_faces == 0) return VOX_OK;
+    pack_kernel<<<grid_for(n_faces), 256, 0, (hipStream_t)stream>>>(d_vertices, n_vertices, d_faces, n_faces, d_tris,
+                                                                    (unsigned long long*)d_bad);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? VOX_OK : hip_fail(e, "pack kernel");
+}
+
+vox_status vox_upload_async(const void* host, size_t bytes, void* d_dst, void* stream) {
+    if (bytes && (!host || !d_dst)) return fail(VOX_ERR_INVALID, "NULL argument");
+    hipStream_t st = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lock(g_up_mutex);
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 2 && e == hipSuccess; i++) {
+        if (!g_up_buf[i]) e = hipHostMalloc(&g_up_buf[i], kUpChunk, hipHostMallocDefault);
+        if (e == hipSuccess && !g_up_ev[i]) e = hipEventCreateWithFlags(&g_up_ev[i], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) return hip_fail(e, "pinned staging");
+    size_t off = 0;
+    int k = 0;
+    while (off < bytes) {
+        const size_t n = bytes - off < kUpChunk ? bytes - off : kUpChunk;
+        e = hipEventSynchronize(g_up_ev[k]);  // the previous copy out of this buffer is done
+        if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+        std::memcpy(g_up_buf[k], (const char*)host + off, n);
+        e = hipMemcpyAsync((char*)d_dst + off, g_up_buf[k], n, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipEventRecord(g_up_ev[k], st);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
+        off += n;
+        k ^= 1;
+    }
+    return VOX_OK;
+}
+
+vox_status vox_convert_layout(const uint32_t* d_src, uint32_t* d_dst, uint32_t grid, int to_morton, void* stream) {
+    if (!d_src || !d_dst || d_src == d_dst) return fail(VOX_ERR_INVALID, "need distinct source and destination");
+    if (grid < 4 || (grid & (grid - 1)) || grid > kMaxGrid)
+        return fail(VOX_ERR_UNSUPPORTED, "layout conversion needs a power-of-two grid >= 4");
+    const uint64_t words = ((uint64_t)grid * grid * grid) >> 5;
+    convert_kernel<<<grid_for(words), 256, 0, (hipStream_t)stream>>>(d_src, d_dst, grid, to_morton ? 1 : 0);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? VOX_OK : hip_fail(e, "convert kernel");
+}
+
+vox_status vox_count_voxels(const uint32_t* d_table, uint64_t words, uint64_t* d_count, void* stream) {
+    if (!d_count || (words && !d_table)) return fail(VOX_ERR_INVALID, "NULL argument");
+    if (words == 0) return VOX_OK;
+    count_kernel<<<grid_for(words), 256, 0, (hipStream_t)stream>>>(d_table, words, (unsigned long long*)d_count);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? VOX_OK : hip_fail(e, "count kernel");
+}
+
+int vox_check_voxel(const uint32_t* table, uint32_t grid, uint32_t x, uint32_t y, uint32_t z, int layout) {
+    if (!table || x >= grid || y >= grid || z >= grid) return 0;
+    uint64_t idx;
+    if (layout == VOX_LAYOUT_MORTON) {
+        idx = 0;
+        for (int b = 0; b < 21; b++)
+            idx |= ((uint64_t)((x >> b) & 1u) << (3 * b)) | ((uint64_t)((y >> b) & 1u) << (3 * b + 1)) |
+                   ((uint64_t)((z >> b) & 1u) << (3 * b + 2));
+    } else {
+        idx = (uint64_t)x + (uint64_t)y * grid + (uint64_t)z * grid * grid;
+    }
+    return (int)((table[idx >> 5] >> (31u - (uint32_t)(idx & 31u))) & 1u);
+}
+
+}  // extern "C"

@@ -26,12 +26,50 @@ __device__ __forceinline__ uint32_t spread10(uint32_t v) {
     return v;
 }
 
-// Coefficients a row needs, as published to LDS (38 words).
+// Coefficients a row needs, as published to LDS (42 words).
 struct SurfRec {
     float n[3], d1, d2;
     float xy[9], zx[9], yz[9];  // a[3], b[3], d[3] per projection
     int xa, xb, ya, yb, za, zb;  // voxel box to cover (inclusive, possibly clipped)
+    int nk;                      // x chunks enumerated per row (see plane_band)
+    int band;                    // 1: prune x with the plane band
+    float sx;                    // |n.x| * (xb + 1) * unit.x, the x part of the magnitude scale
+    int pad;
 };
+
+// Conservative x range of a row (y, z) outside of which the plane test
+// (n.p + d1) * (n.p + d2) > 0 certainly rejects.  In exact arithmetic the test
+// passes iff T = n.x*x*ux + C lies in [-max(d1,d2), -min(d1,d2)] (C = n.y*py
+// + n.z*pz; the sign of a float sum is the sign of the exact sum, and a
+// product can only underflow to zero within ~1e-41 of the band edge).  The
+// float evaluation of T is within a few ulp of the magnitudes involved; the
+// margin allows 64 ulp plus two voxels on each side.  Every voxel inside the
+// range is still decided by the exact float tests.
+__device__ __forceinline__ void plane_band(const SurfRec& R, float ux, float py, float pz, int& lo, int& hi) {
+    const double s = (double)R.n[0] * (double)ux;
+    const double C = (double)R.n[1] * (double)py + (double)R.n[2] * (double)pz;
+    const double dmax = R.d1 > R.d2 ? (double)R.d1 : (double)R.d2;
+    const double dmin = R.d1 > R.d2 ? (double)R.d2 : (double)R.d1;
+    const double S = (double)R.sx + fabs((double)R.n[1] * (double)py) + fabs((double)R.n[2] * (double)pz) +
+                     fabs(dmax) + fabs(dmin);
+    const double M = 64.0 * 5.9604644775390625e-08 * S + 1e-30;
+    double a = (-dmax - M - C) / s, b = (-dmin + M - C) / s;
+    if (a > b) {
+        const double t = a;
+        a = b;
+        b = t;
+    }
+    lo = R.xa;
+    hi = R.xb;
+    if (!(a <= b)) return;  // not a number: no pruning
+    // every compare below happens in double before any conversion to int
+    if (a > (double)R.xb) { lo = R.xb + 1; return; }  // band entirely right of the box
+    if (b < (double)R.xa) { hi = R.xa - 1; return; }  // entirely left
+    if (a > (double)R.xa) lo = (int)floor(a) - 2;
+    if (b < (double)R.xb) hi = (int)ceil(b) + 2;
+    if (lo < R.xa) lo = R.xa;
+    if (hi > R.xb) hi = R.xb;
+}
 
 struct WaveRaster {
     SurfRec rec[64];
@@ -39,7 +77,7 @@ struct WaveRaster {
 };
 
 __device__ __forceinline__ void publish(SurfRec& R, const SurfTri& T, int xa, int xb, int ya, int yb, int za,
-                                        int zb) {
+                                        int zb, int nk, int band, float sx) {
 #pragma unroll
     for (int k = 0; k < 3; k++) R.n[k] = T.n[k];
     R.d1 = T.d1;
@@ -51,6 +89,35 @@ __device__ __forceinline__ void publish(SurfRec& R, const SurfTri& T, int xa, in
         R.yz[i] = T.yz.a[i]; R.yz[3 + i] = T.yz.b[i]; R.yz[6 + i] = T.yz.d[i];
     }
     R.xa = xa; R.xb = xb; R.ya = ya; R.yb = yb; R.za = za; R.zb = zb;
+    R.nk = nk;
+    R.band = band;
+    R.sx = sx;
+}
+
+// Chunks per row to enumerate: all x chunks of the box, or -- when the plane
+// band is narrow -- only as many as the widest band of any row can touch.
+__device__ __forceinline__ void band_chunks(const SurfTri& T, const vox_info& info, int xa, int xb, int yb, int zb,
+                                            int& nk, int& band, float& sx) {
+    const int nch = (xb >> 5) - (xa >> 5) + 1;
+    nk = nch;
+    band = 0;
+    sx = 0.0f;
+    const float ax = fabsf(T.n[0]);
+    if (!(ax > 0.0f) || !isfinite(T.n[1]) || !isfinite(T.n[2]) || !isfinite(T.d1) || !isfinite(T.d2)) return;
+    sx = ax * (float)(xb + 1) * info.unit[0] * 1.0001f;
+    const double s = (double)ax * (double)info.unit[0];
+    const double Smax = (double)sx + fabs((double)T.n[1]) * (yb + 1.0) * info.unit[1] +
+                        fabs((double)T.n[2]) * (zb + 1.0) * info.unit[2] + fabs((double)T.d1) + fabs((double)T.d2);
+    const double M = 64.0 * 5.9604644775390625e-08 * Smax + 1e-30;
+    const double width = (fabs((double)T.d1 - (double)T.d2) + 2.0 * M) / s + 6.0;  // voxels, with the +-2 slack
+    if (!(width < 1e9)) return;
+    const int k = (int)ceil(width / 32.0) + 1;
+    if (k < nch) {
+        nk = k;
+        band = 1;
+    } else {
+        band = 1;  // still prune x inside the enumerated chunks
+    }
 }
 
 __device__ __forceinline__ bool pass3(const float* E, float pu, float pv) {
@@ -65,11 +132,10 @@ __device__ __forceinline__ bool pass3(const float* E, float pu, float pv) {
     return ok;
 }
 
-// Items of a box: rows x 32-voxel chunks aligned to multiples of 32 in x.
-__device__ __forceinline__ unsigned box_items(int xa, int xb, int ya, int yb, int za, int zb) {
+// Items of a box: rows x (enumerated) 32-voxel chunks.
+__device__ __forceinline__ unsigned box_items(int xa, int xb, int ya, int yb, int za, int zb, int nk) {
     if (xa > xb || ya > yb || za > zb) return 0u;
-    const unsigned nch = (unsigned)((xb >> 5) - (xa >> 5) + 1);
-    return (unsigned)(yb - ya + 1) * (unsigned)(zb - za + 1) * nch;
+    return (unsigned)(yb - ya + 1) * (unsigned)(zb - za + 1) * (unsigned)nk;
 }
 
 // Rasterize the wave's triangles.  `have` says whether this lane brings a
@@ -81,8 +147,11 @@ template <class Emit>
 __device__ __forceinline__ void wave_raster(WaveRaster& W, const vox_info& info, bool have, const SurfTri& T,
                                             int xa, int xb, int ya, int yb, int za, int zb, Emit&& emit) {
     const int lane = __lane_id();
-    unsigned items = have ? box_items(xa, xb, ya, yb, za, zb) : 0u;
-    if (items) publish(W.rec[lane], T, xa, xb, ya, yb, za, zb);
+    int nk = 1, band = 0;
+    float sx = 0.0f;
+    if (have && xa <= xb) band_chunks(T, info, xa, xb, yb, zb, nk, band, sx);
+    unsigned items = have ? box_items(xa, xb, ya, yb, za, zb, nk) : 0u;
+    if (items) publish(W.rec[lane], T, xa, xb, ya, yb, za, zb, nk, band, sx);
     // inclusive scan of item counts over the wave
     unsigned incl = items;
 #pragma unroll
@@ -103,15 +172,20 @@ __device__ __forceinline__ void wave_raster(WaveRaster& W, const vox_info& info,
         }
         const SurfRec& R = W.rec[lo];
         const unsigned q = j - (lo ? W.incl[lo - 1] : 0u);
-        const int c0 = R.xa >> 5;
-        const unsigned nch = (unsigned)((R.xb >> 5) - c0 + 1);
+        const unsigned nk = (unsigned)R.nk;
         const unsigned ny = (unsigned)(R.yb - R.ya + 1);
-        const unsigned row = q / nch, ch = q - row * nch;
+        const unsigned row = q / nk, k = q - row * nk;
         const int y = R.ya + (int)(row % ny), z = R.za + (int)(row / ny);
         const float py = (float)y * uy, pz = (float)z * uz;
+        int bx0 = R.xa, bx1 = R.xb;
+        if (R.band) {
+            plane_band(R, ux, py, pz, bx0, bx1);
+            if (bx0 > bx1) continue;
+        }
+        const int cx = ((bx0 >> 5) + (int)k) << 5;
+        if (cx > bx1) continue;
         if (!pass3(R.yz, py, pz)) continue;
-        const int cx = (c0 + (int)ch) << 5;
-        const int x0 = cx > R.xa ? cx : R.xa, x1 = (cx + 31) < R.xb ? (cx + 31) : R.xb;
+        const int x0 = cx > bx0 ? cx : bx0, x1 = (cx + 31) < bx1 ? (cx + 31) : bx1;
         // row-invariant parts of the plane, XY and ZX tests
         const float nypy = R.n[1] * py, nzpz = R.n[2] * pz;
         float bxy[3], azx[3];

@@ -103,6 +103,25 @@ vox_status vox_surface(const vox_info* info, const float* d_tris, uint32_t* d_ta
 vox_status vox_solid(const vox_info* info, const float* d_tris, uint32_t* d_table, int layout,
                      unsigned flags, uint64_t* d_offrow, void* stream);
 
+/* ---- mesh preparation and table utilities (the steps either side of the path) --- */
+/* Component-wise min/max of n_vertices xyz vertices (trimesh2 need_bbox, used at
+   main.cpp:179), written to d_out6 = {min.xyz, max.xyz} (device memory). */
+vox_status vox_mesh_bbox(const float* d_vertices, uint64_t n_vertices, float* d_out6, void* stream);
+/* Indexed mesh -> AoS triangles d_tris[9*n_faces] (meshToGPU_managed,
+   main.cpp:61-80).  Face indices outside [0, n_vertices) yield a zero vertex and
+   are counted into *d_bad (added to; may be NULL). */
+vox_status vox_pack_triangles(const float* d_vertices, uint64_t n_vertices, const int32_t* d_faces,
+                              uint64_t n_faces, float* d_tris, uint64_t* d_bad, void* stream);
+/* Host -> device copy through double-buffered pinned staging (returns once the
+   last chunk is queued; the copies are ordered on `stream`). */
+vox_status vox_upload_async(const void* host, size_t bytes, void* d_dst, void* stream);
+/* Re-layout a power-of-two grid table: linear -> Morton (to_morton=1) or back. */
+vox_status vox_convert_layout(const uint32_t* d_src, uint32_t* d_dst, uint32_t grid, int to_morton, void* stream);
+/* Set voxels of a table of `words` words, added to *d_count (device). */
+vox_status vox_count_voxels(const uint32_t* d_table, uint64_t words, uint64_t* d_count, void* stream);
+/* checkVoxel (util.h:25-38) on a HOST copy of a table, either layout: 1 if set. */
+int vox_check_voxel(const uint32_t* table, uint32_t grid, uint32_t x, uint32_t y, uint32_t z, int layout);
+
 /* ---- batched mode (no counterpart in the reference) ---------------------------- */
 /* Many voxelizations -- meshes of any size, grids, layouts, surface or solid --
    captured once into a HIP graph in which every kernel processes all jobs of

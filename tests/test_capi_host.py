@@ -126,3 +126,16 @@ def test_tile_plans_respect_kernel_bounds(capi):
                     assert layout == 1 and solid and tx == g and ty == tz and ty ** 3 >= 32 and roww == g // 32 <= 64
                     assert words == roww * ty * tz
     assert seen > 20
+
+
+def test_check_voxel_matches_layout_decoding(oracle):
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import meshgen
+    v, f = meshgen.icosphere_jitter(level=2, seed=3)
+    info = oracle.info_for_mesh(v, 16, f.shape[0])
+    for morton in (False, True):
+        t, _ = oracle.surface(info, (v, f), morton=morton)
+        occ = oracle.occupancy(t, 16, morton=morton)
+        for (z, y, x) in [(0, 0, 0), (15, 15, 15)] + [tuple(p) for p in np.argwhere(occ)[:50]]:
+            assert V.check_voxel(t, 16, int(x), int(y), int(z), morton) == bool(occ[z, y, x])
+        assert not V.check_voxel(t, 16, 16, 0, 0, morton)  # out of range reads nothing

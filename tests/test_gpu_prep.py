@@ -1,0 +1,45 @@
+"""GPU tests of the steps either side of the path: device bbox and packing
+against the host definitions, layout conversion round trips, voxel counts."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_prepare_mesh_matches_host_prep(oracle, gpu):
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import meshgen
+    v, f = meshgen.icosphere_sh(level=5, seed=5)
+    v = v + np.float32(-0.37)  # negative coordinates exercise the ordered-key min/max
+    info, tris = V.prepare_mesh(v, f, 128)
+    host_info = V.voxinfo_for_mesh(v, 128, f.shape[0])
+    assert info.as_bytes() == host_info.as_bytes()
+    np.testing.assert_array_equal(tris.cpu().numpy().reshape(-1, 3, 3), meshgen.to_aos(v, f))
+    got = V.voxelize_surface(info, tris).cpu().numpy().view(np.uint32)
+    exp, _ = oracle.surface(oracle.info_from_bytes(info.as_bytes()), (v, f))
+    assert (got == exp).all()
+
+
+def test_pack_rejects_bad_indices(gpu):
+    import torch
+    import cuda_voxelizer_amd as V
+    vtx = torch.zeros(3 * 4, dtype=torch.float32, device=gpu)
+    faces = torch.tensor([0, 1, 2, 1, 2, 7], dtype=torch.int32, device=gpu)
+    with pytest.raises(ValueError):
+        V.pack_triangles(vtx, faces)
+
+
+@pytest.mark.parametrize("grid", [4, 32, 64])
+def test_layout_conversion_round_trip(oracle, gpu, grid):
+    import torch
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import meshgen
+    v, f = meshgen.icosphere_jitter(level=3, seed=grid)
+    info = V.voxinfo_for_mesh(v, grid, f.shape[0])
+    d = torch.from_numpy(meshgen.to_aos(v, f).reshape(-1)).to(gpu)
+    lin = V.voxelize_solid_table(info, d)
+    mor = V.voxelize_solid_table(info, d, morton=True)
+    assert (V.convert_layout(lin, grid, True) == mor).all()
+    assert (V.convert_layout(mor, grid, False) == lin).all()
+    n = V.count_voxels(lin)
+    assert n == V.count_voxels(mor) == int(np.unpackbits(lin.cpu().numpy().view(np.uint8)).sum())
