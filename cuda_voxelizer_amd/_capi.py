@@ -55,6 +55,17 @@ class vox_info(ctypes.Structure):
 assert ctypes.sizeof(vox_info) == 64
 
 
+class vox_mesh(ctypes.Structure):
+    _fields_ = [("vertices", ctypes.POINTER(ctypes.c_float)), ("n_vertices", ctypes.c_uint64),
+                ("faces", ctypes.POINTER(ctypes.c_int32)), ("n_faces", ctypes.c_uint64)]
+
+
+VOX_ENC_BINVOX = 1
+VOX_ENC_OBJ_POINTS = 2
+VOX_ENC_OBJ_CUBES = 3
+VOX_ENC_VOX = 4
+
+
 class vox_batch_item(ctypes.Structure):
     _fields_ = [
         ("info", vox_info),
@@ -99,6 +110,15 @@ SIGNATURES = {
     "vox_runner_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "vox_runner_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(vox_batch_item), ctypes.c_int]),
     "vox_runner_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "vox_encoder_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "vox_encode": (ctypes.c_int, [ctypes.c_void_p, _P, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_uint64)]),
+    "vox_encoder_data": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "vox_encoder_copy": (ctypes.c_int, [ctypes.c_void_p, _P, ctypes.c_uint64]),
+    "vox_encoder_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "vox_binvox_header": (ctypes.c_int, [ctypes.POINTER(vox_info), ctypes.c_char_p, ctypes.c_size_t]),
+    "vox_load_mesh": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(vox_mesh)]),
+    "vox_free_mesh": (None, [ctypes.POINTER(vox_mesh)]),
 }
 
 _lib = None

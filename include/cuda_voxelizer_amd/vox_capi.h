@@ -164,6 +164,59 @@ vox_status vox_runner_create(void* stream, vox_runner** out);
 vox_status vox_runner_launch(vox_runner* runner, const vox_batch_item* items, int count);
 vox_status vox_runner_destroy(vox_runner* runner);
 
+/* ---- mesh files (the reference reads them with trimesh2, main.cpp:174-175) ------ */
+/* Host arrays: vertices float[3*n_vertices], faces int32[3*n_faces]. */
+typedef struct vox_mesh {
+    float* vertices;
+    uint64_t n_vertices;
+    int32_t* faces;
+    uint64_t n_faces;
+} vox_mesh;
+/* Loads .obj / .off / .stl / .ply (by extension) into *out (polygons split
+   into fans around their first vertex; coordinates rounded once to binary32).
+   Every face index is checked against the vertex count. */
+vox_status vox_load_mesh(const char* path, vox_mesh* out);
+void vox_free_mesh(vox_mesh* mesh);
+
+/* ---- output encoders (the reference's writers, util_io.cpp) on the device ------- */
+/* The reference writes its outputs on the host with one checkVoxel per voxel.
+   An encoder permutes the table into the writer's traversal order, counts,
+   scans and emits the file body on the device; the caller copies the bytes out.
+     VOX_ENC_BINVOX      binvox RLE body: (value, count) byte pairs, y fastest,
+                         then z, then x; a new pair after 255 equal voxels
+                         (write_binvox, util_io.cpp:214-263; header below)
+     VOX_ENC_OBJ_POINTS  "v x+0.5 y+0.5 z+0.5" per set voxel, z fastest, then y,
+                         then x (write_obj_pointcloud, util_io.cpp:149-186)
+     VOX_ENC_OBJ_CUBES   8 "v" + 12 "f" lines per set voxel in the same order
+                         (write_cube / write_obj_cubes, util_io.cpp:44-123; the
+                         reference then re-writes the file through trimesh2's
+                         reorder_verts, which is not reproduced)
+   Tables are cubic, either layout (Morton needs a power-of-two grid >= 4),
+   grid edge 1..2048.  The morton output of the reference (write_binary of the
+   raw table, util_io.cpp:188-196) needs no encoder: copy the table. */
+#define VOX_ENC_BINVOX 1
+#define VOX_ENC_OBJ_POINTS 2
+#define VOX_ENC_OBJ_CUBES 3
+#define VOX_ENC_VOX 4 /* complete MagicaVoxel .vox file (write_vox, util_io.cpp:266-300, through
+                         the vendored VoxWriter: 126^3 cubes, XYZI chunks, scene graph, palette) */
+typedef struct vox_encoder vox_encoder;
+/* An encoder bound to a stream; it keeps grow-only device buffers. */
+vox_status vox_encoder_create(void* stream, vox_encoder** out);
+/* Encodes d_table (device memory) on the encoder's stream and waits for it
+   once, to size the output; *out_bytes = length of the encoded body. */
+vox_status vox_encode(vox_encoder* enc, const uint32_t* d_table, uint32_t grid, int layout, int format,
+                      uint64_t* out_bytes);
+/* Device pointer of the last encoded body (valid until the next vox_encode or
+   vox_encoder_destroy on this encoder). */
+vox_status vox_encoder_data(vox_encoder* enc, const uint8_t** d_bytes);
+/* Copies the first `bytes` bytes of the last encoded body to host memory. */
+vox_status vox_encoder_copy(vox_encoder* enc, void* host, uint64_t bytes);
+vox_status vox_encoder_destroy(vox_encoder* enc);
+/* The binvox ASCII header (util_io.cpp:219-226: dim, translate = bbox min,
+   scale = largest bbox extent, %g formatting) into buf; returns its length as
+   snprintf does, or -1 if info is NULL. */
+int vox_binvox_header(const vox_info* info, char* buf, size_t cap);
+
 /* ---- introspection ----------------------------------------------------------- */
 /* Output tiling the tile algorithm uses for a grid/layout/mode (no counterpart
    in the reference).  out[0..7] = kind (0 linear rows, 1 Morton cubes, 2 Morton

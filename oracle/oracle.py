@@ -62,6 +62,9 @@ def load():
     lib.voxo_sizeof_info.restype = ctypes.c_int
     lib.voxo_set_threads.argtypes = [ctypes.c_int]
     lib.voxo_max_threads.restype = ctypes.c_int
+    for fn in ("voxo_binvox_body", "voxo_obj_points", "voxo_obj_cubes", "voxo_vox_file"):
+        getattr(lib, fn).restype = ctypes.c_uint64
+        getattr(lib, fn).argtypes = [P, ctypes.c_uint32, ctypes.c_int, P, ctypes.c_uint64]
     _lib = lib
     return lib
 
@@ -187,3 +190,33 @@ def occupancy(table: np.ndarray, grid: int, morton: bool = False) -> np.ndarray:
         code |= ((y.astype(np.uint64) >> b) & 1) << (3 * b + 1)
         code |= ((z.astype(np.uint64) >> b) & 1) << (3 * b + 2)
     return bits[code.astype(np.int64)]
+
+
+def _writer(name: str, table: np.ndarray, grid: int, morton: bool) -> bytes:
+    lib = load()
+    t = np.ascontiguousarray(table).view(np.uint32)
+    fn = getattr(lib, name)
+    n = fn(_ptr(t), grid, 1 if morton else 0, None, 0)
+    out = np.zeros(max(1, n), np.uint8)
+    fn(_ptr(t), grid, 1 if morton else 0, _ptr(out), n)
+    return out[:n].tobytes()
+
+
+def binvox_body(table: np.ndarray, grid: int, morton: bool = False) -> bytes:
+    """RLE body of write_binvox (util_io.cpp:228-262)."""
+    return _writer("voxo_binvox_body", table, grid, morton)
+
+
+def obj_points(table: np.ndarray, grid: int, morton: bool = False) -> bytes:
+    """Text of write_obj_pointcloud (util_io.cpp:149-186)."""
+    return _writer("voxo_obj_points", table, grid, morton)
+
+
+def obj_cubes(table: np.ndarray, grid: int, morton: bool = False) -> bytes:
+    """Text of write_obj_cubes before its trimesh2 re-write (util_io.cpp:44-123)."""
+    return _writer("voxo_obj_cubes", table, grid, morton)
+
+
+def vox_file(table: np.ndarray, grid: int, morton: bool = False) -> bytes:
+    """MagicaVoxel file of write_vox (util_io.cpp:266-300, VoxWriter)."""
+    return _writer("voxo_vox_file", table, grid, morton)

@@ -132,6 +132,7 @@ static void build_luts(void) {
    the true interleave only for coordinates < 2^16. */
 uint64_t voxo_morton_encode(uint32_t x, uint32_t y, uint32_t z) {
     uint64_t answer = 0;
+    build_luts();
     answer = (uint64_t)(lut_z[(z >> 16) & 0xFF] | lut_y[(y >> 16) & 0xFF] | lut_x[(x >> 16) & 0xFF]);
     answer = answer << 48 | lut_z[(z >> 8) & 0xFF] | lut_y[(y >> 8) & 0xFF] | lut_x[(x >> 8) & 0xFF];
     answer = answer << 24 | lut_z[z & 0xFF] | lut_y[y & 0xFF] | lut_x[x & 0xFF];

@@ -134,3 +134,15 @@ def solid_parity(tris_translated: np.ndarray, unit: np.ndarray, grid: int, tol: 
             if xm >= 0:
                 par[zz, yy, : min(xm, G - 1) + 1] ^= True
     return par, tiecol
+
+
+def morton_index(x, y, z):
+    """Bit interleave x0 y0 z0 x1 y1 z1 ... of non-negative integer arrays (16 bits each)."""
+    x, y, z = (np.asarray(a, np.uint64) for a in (x, y, z))
+    out = np.zeros(np.broadcast(x, y, z).shape, np.uint64)
+    for b in range(16):
+        ub = np.uint64(b)
+        out |= ((x >> ub) & np.uint64(1)) << np.uint64(3 * b)
+        out |= ((y >> ub) & np.uint64(1)) << np.uint64(3 * b + 1)
+        out |= ((z >> ub) & np.uint64(1)) << np.uint64(3 * b + 2)
+    return out.astype(np.int64)
