@@ -36,7 +36,7 @@ __all__ = [
     "prepare_mesh",
 ]
 
-ALGOS = tuple(ALGO_FLAGS)  # "auto", "direct", "tile", "chunk"
+ALGOS = tuple(ALGO_FLAGS)  # "auto", "direct", "tile", "chunk", "marker"
 
 
 def _algo_flag(algo: str) -> int:

@@ -25,7 +25,9 @@ VOX_FLAG_ACCUMULATE = 0x1
 VOX_FLAG_ALGO_DIRECT = 0x2
 VOX_FLAG_ALGO_TILE = 0x4
 VOX_FLAG_ALGO_CHUNK = 0x8
-ALGO_FLAGS = {"auto": 0, "direct": VOX_FLAG_ALGO_DIRECT, "tile": VOX_FLAG_ALGO_TILE, "chunk": VOX_FLAG_ALGO_CHUNK}
+VOX_FLAG_ALGO_MARKER = 0x10
+ALGO_FLAGS = {"auto": 0, "direct": VOX_FLAG_ALGO_DIRECT, "tile": VOX_FLAG_ALGO_TILE, "chunk": VOX_FLAG_ALGO_CHUNK,
+              "marker": VOX_FLAG_ALGO_MARKER}
 
 _STATUS_NAMES = {0: "VOX_OK", 1: "VOX_ERR_INVALID", 2: "VOX_ERR_HIP", 3: "VOX_ERR_OOM",
                  4: "VOX_ERR_UNSUPPORTED", 5: "VOX_ERR_NO_DEVICE"}

@@ -119,21 +119,31 @@ vox_status run(const vox_info* info, const float* d_tris, uint32_t* d_table, int
                uint64_t* d_offrow, void* stream, bool solid) {
     vox_status s = validate(info, d_tris, d_table, layout);
     if (s != VOX_OK) return s;
-    const unsigned algo_bits = flags & (VOX_FLAG_ALGO_DIRECT | VOX_FLAG_ALGO_TILE | VOX_FLAG_ALGO_CHUNK);
+    const unsigned algo_bits =
+        flags & (VOX_FLAG_ALGO_DIRECT | VOX_FLAG_ALGO_TILE | VOX_FLAG_ALGO_CHUNK | VOX_FLAG_ALGO_MARKER);
     if (algo_bits & (algo_bits - 1)) return fail(VOX_ERR_INVALID, "at most one VOX_FLAG_ALGO_* flag");
     if (solid && (flags & VOX_FLAG_ALGO_CHUNK)) return fail(VOX_ERR_INVALID, "the chunk algorithm is surface-only");
+    if (!solid && (flags & VOX_FLAG_ALGO_MARKER))
+        return fail(VOX_ERR_INVALID, "the marker algorithm is solid-only");
     hipStream_t st = (hipStream_t)stream;
     const uint32_t G = info->gridsize[0];
     const bool accumulate = (flags & VOX_FLAG_ACCUMULATE) != 0;
     TilePlan P;
     const bool have_plan = make_plan((int)G, layout, solid, P);
-    enum { DIRECT, TILE, CHUNK } algo;
+    const bool have_marker = solid && marker_supported(G, layout, accumulate);
+    enum { DIRECT, TILE, CHUNK, MARKER } algo;
     if (flags & VOX_FLAG_ALGO_DIRECT) algo = DIRECT;
     else if (flags & VOX_FLAG_ALGO_TILE) algo = have_plan ? TILE : DIRECT;
     else if (flags & VOX_FLAG_ALGO_CHUNK) algo = chunk_supported(G) ? CHUNK : DIRECT;
-    else if (solid) algo = have_plan ? TILE : DIRECT;
+    else if (flags & VOX_FLAG_ALGO_MARKER) algo = have_marker ? MARKER : (have_plan ? TILE : DIRECT);
+    else if (solid) algo = have_marker ? MARKER : (have_plan ? TILE : DIRECT);
     else algo = chunk_supported(G) ? CHUNK : (have_plan ? TILE : DIRECT);
     hipError_t e;
+    if (algo == MARKER) {
+        e = launch_solid_marker(*info, d_tris, d_table, (unsigned long long*)d_offrow, st);
+        if (e != hipSuccess) return hip_fail(e, "marker kernels launch");
+        return VOX_OK;
+    }
     if (algo == TILE) {
         void* ws = nullptr;
         WorkspaceLayout L;

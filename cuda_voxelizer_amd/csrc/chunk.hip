@@ -1,15 +1,17 @@
 // chunk.hip -- single-pass surface voxelization with LDS word merging.
 //
-// One workgroup takes a chunk of consecutive triangles (meshes are stored
+// One workgroup takes a chunk of 256 consecutive triangles (meshes are stored
 // with spatially coherent triangle order, so a chunk covers a compact
-// region).  The chunk is staged through LDS with coalesced loads, each wave
-// rasterizes its 64 triangles with the wave-balanced raster (raster.h), and
-// the passing voxels are OR-ed into an LDS hash table keyed by table word, so
-// bits that several triangles of the chunk set in one word are merged before
-// memory is touched.  The table is then updated with one non-returning global
-// atomicOr per distinct word; words that do not fit in the hash table go
-// straight to global atomics.  The table is zero-filled beforehand (unless
-// accumulating).
+// region).  The chunk is staged through LDS with coalesced loads; each lane
+// rasterizes its own triangle row by row (the YZ test once per row, then the
+// plane, XY and ZX tests per voxel; raster.h row_mask) and ORs the passing
+// bits into an LDS hash table keyed by table word, so bits that several
+// triangles of the chunk set in one word are merged before memory is
+// touched.  Triangles with a large voxel box are queued and their rows split
+// over the whole workgroup afterwards.  The table is then updated with one
+// non-returning global atomicOr per distinct word; words that do not fit in
+// the hash go straight to global atomics.  The table is zero-filled
+// beforehand (unless accumulating).
 #include "jobs.h"
 #include "raster.h"
 #include "vox_kernels.h"
@@ -19,23 +21,24 @@
 namespace voxamd {
 
 constexpr int kChunk = 256;  // triangles per workgroup (one per lane)
-constexpr int kHashBits = 10;
+constexpr int kHashBits = 11;
 constexpr int kHashCap = 1 << kHashBits;
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr int kQueue = 32;      // big triangles split over the workgroup
+constexpr int kBigVox = 256;    // voxel boxes above this (or wider than 32 in x) are queued
 
 struct ChunkShared {
     uint32_t keys[kHashCap];
     uint32_t vals[kHashCap];
-    union {
-        float tri[kChunk * 9];     // staging of the chunk's triangles
-        WaveRaster wave[kChunk / 64];
-    } u;
+    float tri[kChunk * 9];
+    unsigned queue[kQueue];
+    unsigned qn;
 };
 
 __device__ __forceinline__ void word_or(ChunkShared& S, uint32_t* __restrict__ table, uint32_t key, uint32_t mask) {
     uint32_t h = (key * 2654435761u) >> (32 - kHashBits);
 #pragma unroll 1
-    for (int probe = 0; probe < 8; probe++) {
+    for (int probe = 0; probe < 16; probe++) {
         uint32_t k = S.keys[h];
         if (k == kEmpty) {
             k = atomicCAS(&S.keys[h], kEmpty, key);
@@ -50,77 +53,143 @@ __device__ __forceinline__ void word_or(ChunkShared& S, uint32_t* __restrict__ t
     atomicOr(table + key, mask);  // hash table crowded: go to memory directly
 }
 
-// Work unit: kChunk consecutive triangles of one job.
+// Row (y, z) voxels x0..x1 inside the 32-voxel chunk at cx -> hash.
+__device__ __forceinline__ void chunk_row(ChunkShared& S, uint32_t* __restrict__ table, const SurfTri& T,
+                                          const float* xy, const float* zx, const vox_info& info, int layout,
+                                          int x0, int x1, int cx, int y, int z, float py, float pz) {
+    const uint32_t G = info.gridsize[0];
+    const uint32_t mask = row_mask(T.n, T.d1, T.d2, xy, zx, info.unit[0], py, pz, x0, x1, cx);
+    if (!mask) return;
+    if (layout == VOX_LAYOUT_MORTON) {
+        // codes of grids up to 1024 fit 30 bits: 32-bit spreads
+        const uint64_t base = G <= 1024u ? (uint64_t)(spread10((uint32_t)cx) | (spread10((uint32_t)y) << 1) |
+                                                      (spread10((uint32_t)z) << 2))
+                                         : morton3((uint32_t)cx, (uint32_t)y, (uint32_t)z);
+        const uint32_t yz_low = ((uint32_t)(y & 1) << 1) | ((uint32_t)(z & 1) << 2) | ((uint32_t)((y >> 1) & 1) << 4);
+        morton_chunk_words(base, yz_low, mask, [&](uint64_t w, uint32_t bits) { word_or(S, table, (uint32_t)w, bits); });
+    } else if ((G & 31u) == 0) {
+        const uint64_t w = ((uint64_t)cx + (uint64_t)y * G + (uint64_t)z * G * G) >> 5;
+        word_or(S, table, (uint32_t)w, __brev(mask));
+    } else {
+        // linear rows not aligned to words: per voxel
+        const uint64_t row = (uint64_t)y * G + (uint64_t)z * G * G;
+        uint32_t m = mask, cur = kEmpty, bits = 0;
+        while (m) {
+            const int k = __ffs(m) - 1;
+            m &= m - 1;
+            const uint64_t idx = row + (uint64_t)(cx + k);
+            const uint32_t w = (uint32_t)(idx >> 5);
+            if (w != cur) {
+                if (bits) word_or(S, table, cur, bits);
+                cur = w;
+                bits = 0;
+            }
+            bits |= bit_mask(idx);
+        }
+        if (bits) word_or(S, table, cur, bits);
+    }
+}
+
+__device__ __forceinline__ void tri_coeffs(const SurfTri& T, float xy[9], float zx[9]) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        xy[i] = T.xy.a[i]; xy[3 + i] = T.xy.b[i]; xy[6 + i] = T.xy.d[i];
+        zx[i] = T.zx.a[i]; zx[3 + i] = T.zx.b[i]; zx[6 + i] = T.zx.d[i];
+    }
+}
+
+__device__ __forceinline__ void staged_vertices(const ChunkShared& S, int lt, const vox_info& info, float v[3][3]) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        v[k][0] = S.tri[lt * 9 + 3 * k + 0] - info.bbox_min[0];
+        v[k][1] = S.tri[lt * 9 + 3 * k + 1] - info.bbox_min[1];
+        v[k][2] = S.tri[lt * 9 + 3 * k + 2] - info.bbox_min[2];
+    }
+}
+
+// Work unit: kChunk consecutive triangles of one job.  MULTI: the job comes
+// from the launch's job map; otherwise it is `one` (kernel arguments).
+template <bool MULTI>
 __global__ void __launch_bounds__(kChunk) surface_chunk_kernel(ItemDev one, ItemMap M) {
     __shared__ ChunkShared S;
     __shared__ BlockJob J;
-    resolve_job(one, M, blockIdx.x, J);
-    if (J.local == ~0u) return;
-    const vox_info info = J.it.info;
-    const float* __restrict__ tris = J.it.tris;
-    uint32_t* __restrict__ table = J.it.table;
-    const int layout = J.it.layout;
+    uint32_t local = blockIdx.x;
+    if (MULTI) {
+        resolve_job(one, M, blockIdx.x, J);
+        local = J.local;
+        if (local == ~0u) return;
+    }
+    const ItemDev& it = MULTI ? J.it : one;
+    const vox_info& info = it.info;
+    uint32_t* __restrict__ table = it.table;
+    const int layout = it.layout;
     const uint64_t n = info.n_triangles;
-    const uint32_t G = info.gridsize[0];
-    const uint64_t first = (uint64_t)J.local * kChunk;
+    const uint64_t first = (uint64_t)local * kChunk;
     if (first >= n) return;
     const int count = (int)((n - first) < (uint64_t)kChunk ? (n - first) : (uint64_t)kChunk);
     for (int i = threadIdx.x; i < kHashCap; i += kChunk) {
         S.keys[i] = kEmpty;
         S.vals[i] = 0u;
     }
-    const float* src = tris + first * 9;
-    for (int i = threadIdx.x; i < count * 9; i += kChunk) S.u.tri[i] = src[i];
+    if (threadIdx.x == 0) S.qn = 0;
+    const float* src = it.tris + first * 9;
+    for (int i = threadIdx.x; i < count * 9; i += kChunk) S.tri[i] = src[i];
     __syncthreads();
-    const bool have = (int)threadIdx.x < count;
-    float v[3][3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        v[k][0] = (have ? S.u.tri[threadIdx.x * 9 + 3 * k + 0] : 0.0f) - info.bbox_min[0];
-        v[k][1] = (have ? S.u.tri[threadIdx.x * 9 + 3 * k + 1] : 0.0f) - info.bbox_min[1];
-        v[k][2] = (have ? S.u.tri[threadIdx.x * 9 + 3 * k + 2] : 0.0f) - info.bbox_min[2];
+    if ((int)threadIdx.x < count) {
+        float v[3][3];
+        staged_vertices(S, threadIdx.x, info, v);
+        SurfTri T;
+        surface_setup(v, info, T);
+        const int nx = T.hi[0] - T.lo[0] + 1;
+        const int vol = nx * (T.hi[1] - T.lo[1] + 1) * (T.hi[2] - T.lo[2] + 1);
+        bool queued = false;
+        if (vol > kBigVox || nx > 32) {
+            const unsigned q = atomicAdd(&S.qn, 1u);
+            if (q < (unsigned)kQueue) {
+                S.queue[q] = threadIdx.x;
+                queued = true;
+            }
+        }
+        if (!queued) {
+            float xy[9], zx[9];
+            tri_coeffs(T, xy, zx);
+            for (int z = T.lo[2]; z <= T.hi[2]; z++) {
+                const float pz = (float)z * info.unit[2];
+                for (int y = T.lo[1]; y <= T.hi[1]; y++) {
+                    const float py = (float)y * info.unit[1];
+                    if (!edges_pass(T.yz, py, pz)) continue;
+                    for (int cx = T.lo[0] & ~31; cx <= T.hi[0]; cx += 32) {
+                        const int x0 = cx > T.lo[0] ? cx : T.lo[0], x1 = cx + 31 < T.hi[0] ? cx + 31 : T.hi[0];
+                        chunk_row(S, table, T, xy, zx, info, layout, x0, x1, cx, y, z, py, pz);
+                    }
+                }
+            }
+        }
     }
-    SurfTri T;
-    surface_setup(v, info, T);
-    __syncthreads();  // staging area is reused by the raster
-    WaveRaster& W = S.u.wave[threadIdx.x >> 6];
-    if (layout == VOX_LAYOUT_LINEAR && (G & 31u) == 0) {
-        // rows are word aligned: one x chunk is one table word
-        wave_raster(W, info, have, T, T.lo[0], T.hi[0], T.lo[1], T.hi[1], T.lo[2], T.hi[2],
-                    [&](int y, int z, int cx, uint32_t mask) {
-                        const uint64_t w = ((uint64_t)cx + (uint64_t)y * G + (uint64_t)z * G * G) >> 5;
-                        word_or(S, table, (uint32_t)w, __brev(mask));
-                    });
-    } else if (layout == VOX_LAYOUT_MORTON) {
-        wave_raster(W, info, have, T, T.lo[0], T.hi[0], T.lo[1], T.hi[1], T.lo[2], T.hi[2],
-                    [&](int y, int z, int cx, uint32_t mask) {
-                        const uint64_t base = morton3((uint32_t)cx, (uint32_t)y, (uint32_t)z);
-                        const uint32_t yz_low = ((uint32_t)(y & 1) << 1) | ((uint32_t)(z & 1) << 2) |
-                                                ((uint32_t)((y >> 1) & 1) << 4);
-                        morton_chunk_words(base, yz_low, mask,
-                                           [&](uint64_t w, uint32_t bits) { word_or(S, table, (uint32_t)w, bits); });
-                    });
-    } else {
-        // linear rows not aligned to words (G % 32 != 0): per voxel
-        wave_raster(W, info, have, T, T.lo[0], T.hi[0], T.lo[1], T.hi[1], T.lo[2], T.hi[2],
-                    [&](int y, int z, int cx, uint32_t mask) {
-                        const uint64_t row = (uint64_t)y * G + (uint64_t)z * G * G;
-                        uint32_t cur = kEmpty, bits = 0;
-                        while (mask) {
-                            const int k = __ffs(mask) - 1;
-                            mask &= mask - 1;
-                            const uint64_t idx = row + (uint64_t)(cx + k);
-                            const uint32_t w = (uint32_t)(idx >> 5);
-                            if (w != cur) {
-                                if (bits) word_or(S, table, cur, bits);
-                                cur = w;
-                                bits = 0;
-                            }
-                            bits |= bit_mask(idx);
-                        }
-                        if (bits) word_or(S, table, cur, bits);
-                    });
+    __syncthreads();
+    // queued triangles: rows x 32-voxel chunks split over the workgroup
+    const unsigned nq = S.qn < (unsigned)kQueue ? S.qn : (unsigned)kQueue;
+    for (unsigned q = 0; q < nq; q++) {
+        float v[3][3];
+        staged_vertices(S, (int)S.queue[q], info, v);
+        SurfTri T;
+        surface_setup(v, info, T);
+        float xy[9], zx[9];
+        tri_coeffs(T, xy, zx);
+        const int ny = T.hi[1] - T.lo[1] + 1, nz = T.hi[2] - T.lo[2] + 1;
+        const int c0 = T.lo[0] >> 5, nch = (T.hi[0] >> 5) - c0 + 1;
+        const int items = ny * nz * nch;
+        for (int i = threadIdx.x; i < items; i += kChunk) {
+            const int ch = i % nch, r = i / nch;
+            const int y = T.lo[1] + r % ny, z = T.lo[2] + r / ny;
+            const float py = (float)y * info.unit[1], pz = (float)z * info.unit[2];
+            if (!edges_pass(T.yz, py, pz)) continue;
+            const int cx = (c0 + ch) << 5;
+            const int x0 = cx > T.lo[0] ? cx : T.lo[0], x1 = cx + 31 < T.hi[0] ? cx + 31 : T.hi[0];
+            chunk_row(S, table, T, xy, zx, info, layout, x0, x1, cx, y, z, py, pz);
+        }
     }
+    __syncthreads();  // every hash insert is done
     for (int i = threadIdx.x; i < kHashCap; i += kChunk) {
         const uint32_t k = S.keys[i];
         if (k != kEmpty) atomicOr(table + k, S.vals[i]);
@@ -168,7 +237,10 @@ hipError_t launch_fill_jobs(const ItemDev& one, const ItemMap& M, uint32_t units
 
 hipError_t launch_chunk_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, hipStream_t stream) {
     if (units == 0) return hipSuccess;
-    surface_chunk_kernel<<<units, kChunk, 0, stream>>>(one, M);
+    if (M.items)
+        surface_chunk_kernel<true><<<units, kChunk, 0, stream>>>(one, M);
+    else
+        surface_chunk_kernel<false><<<units, kChunk, 0, stream>>>(one, M);
     return hipGetLastError();
 }
 

@@ -1,0 +1,310 @@
+// marker.hip -- solid voxelization by run-end markers and a row suffix XOR.
+//
+// The reference XORs, for every hit column (y, z) of every triangle, the
+// whole run x = 0..xmax into the table (cpu_voxelizer.cpp:294-305): O(xmax)
+// bit writes per crossing.  The final bit (x, y, z) is the parity of the runs
+// of its column with xmax >= x, so it is enough to toggle ONE marker bit at
+// (xmax, y, z) per run and then replace every row by its suffix XOR along x:
+//
+//   1. fill      zero the table (fill_zero_kernel)
+//   2. markers   per triangle (one lane each; triangles with many columns
+//                are split over the workgroup): solid setup, column tests,
+//                xmax, one non-returning atomicXor per hit column
+//   3. suffix    per row (G/32 words, lanes = words): in-word prefix XOR from
+//                bit 0 up (larger x = lower bit) plus the parity of the words
+//                of larger x (a ballot), in place
+//   4. spills    runs with xmax >= G continue along the linear index into the
+//                following rows (as the reference indexes them): their
+//                in-row part is a marker at G-1, the rest is recorded and
+//                XORed into the final table afterwards (word masks, atomics)
+//
+// Linear layout with G % 32 == 0 (rows are whole words), table defined by the
+// call (no accumulate).  Per-column arithmetic: vox_device.h.
+#include "vox_device.h"
+#include "vox_host.h"
+#include "vox_kernels.h"
+
+#include <map>
+#include <mutex>
+#include <utility>
+
+#pragma clang fp contract(off)
+
+namespace voxamd {
+
+constexpr int kMBlock = 256;
+constexpr int kMQueue = 32;        // triangles with many columns, split over the workgroup
+constexpr int kMBigCols = 64;      // columns above which a triangle is queued
+constexpr uint32_t kSpillCap = 1u << 16;
+
+// misc words: [0] spill count, [1] spill overflow flag
+struct MarkerWs {
+    unsigned* misc;
+    unsigned long long* spill;  // [2*i] first bit, [2*i+1] last bit (inclusive)
+};
+
+__global__ void __launch_bounds__(64) marker_init_kernel(unsigned* misc) {
+    if (threadIdx.x < 4) misc[threadIdx.x] = 0u;
+}
+
+// One hit column: marker toggle (and spill record) of the run x = 0..xmax.
+__device__ __forceinline__ void column_run(uint32_t* __restrict__ table, int64_t xmax, int y, int z, uint32_t G,
+                                           uint64_t total_bits, const MarkerWs& ws, unsigned long long& offrow) {
+    if (xmax < 0 || xmax >= (int64_t)G) offrow++;
+    if (xmax < 0) return;
+    const uint64_t row = (uint64_t)y * G + (uint64_t)z * G * G;
+    const uint32_t xm = xmax < (int64_t)G ? (uint32_t)xmax : G - 1;
+    const uint64_t idx = row + xm;
+    atomicXor(table + (idx >> 5), bit_mask(idx));
+    if (xmax >= (int64_t)G && row + G < total_bits) {
+        // bits row+G .. row+xmax, clipped to the table
+        const uint64_t last = (uint64_t)xmax < total_bits - 1 - row ? row + (uint64_t)xmax : total_bits - 1;
+        const unsigned slot = atomicAdd(ws.misc, 1u);
+        if (slot < kSpillCap) {
+            ws.spill[2 * slot] = row + G;
+            ws.spill[2 * slot + 1] = last;
+        } else {
+            ws.misc[1] = 1u;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kMBlock) solid_marker_kernel(vox_info info, const float* __restrict__ tris,
+                                                                uint32_t* __restrict__ table, MarkerWs ws,
+                                                                unsigned long long* __restrict__ offrow_out) {
+    __shared__ float s_tri[kMBlock * 9];
+    __shared__ unsigned queue[kMQueue];
+    __shared__ unsigned qn;
+    const uint64_t n = info.n_triangles;
+    const uint64_t first = (uint64_t)blockIdx.x * kMBlock;
+    const int count = (int)((n - first) < (uint64_t)kMBlock ? (n - first) : (uint64_t)kMBlock);
+    const int G = (int)info.gridsize[0];
+    const uint64_t total_bits = (uint64_t)G * G * G;
+    if (threadIdx.x == 0) qn = 0;
+    const float* src = tris + first * 9;
+    for (int i = threadIdx.x; i < count * 9; i += kMBlock) s_tri[i] = src[i];
+    __syncthreads();
+    unsigned long long offrow = 0;
+    if ((int)threadIdx.x < count) {
+        float v[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            v[k][0] = s_tri[threadIdx.x * 9 + 3 * k + 0] - info.bbox_min[0];
+            v[k][1] = s_tri[threadIdx.x * 9 + 3 * k + 1] - info.bbox_min[1];
+            v[k][2] = s_tri[threadIdx.x * 9 + 3 * k + 2] - info.bbox_min[2];
+        }
+        SolidTri S;
+        solid_setup(v, info, S);
+        if (S.valid) {
+            // column range, clamped to the grid, and its out-of-grid events
+            // (as the reference loops from the unclamped bounds)
+            int ylo = (int)S.ymin, zlo = (int)S.zmin;
+            if (ylo < 0) { offrow++; ylo = 0; }
+            if (zlo < 0) { offrow++; zlo = 0; }
+            if (S.ymax > (float)(G - 1)) offrow++;
+            if (S.zmax > (float)(G - 1)) offrow++;
+            const int yhi = S.ymax >= (float)(G - 1) ? G - 1 : (S.ymax >= 0.0f ? (int)S.ymax : -1);
+            const int zhi = S.zmax >= (float)(G - 1) ? G - 1 : (S.zmax >= 0.0f ? (int)S.zmax : -1);
+            const int64_t cols = (yhi >= ylo && zhi >= zlo) ? (int64_t)(yhi - ylo + 1) * (zhi - zlo + 1) : 0;
+            bool queued = false;
+            if (cols > kMBigCols) {
+                const unsigned q = atomicAdd(&qn, 1u);
+                if (q < (unsigned)kMQueue) {
+                    queue[q] = threadIdx.x;
+                    queued = true;
+                }
+            }
+            if (!queued) {
+                for (int y = ylo; y <= yhi; y++) {
+                    const float py = ((float)y + 0.5f) * info.unit[1];
+                    for (int z = zlo; z <= zhi; z++) {
+                        const float pz = ((float)z + 0.5f) * info.unit[2];
+                        if (!column_hit(S, py, pz)) continue;
+                        column_run(table, column_xmax(S, info, py, pz), y, z, (uint32_t)G, total_bits, ws, offrow);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const unsigned nq = qn < (unsigned)kMQueue ? qn : (unsigned)kMQueue;
+    for (unsigned q = 0; q < nq; q++) {
+        const int lt = (int)queue[q];
+        float v[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            v[k][0] = s_tri[lt * 9 + 3 * k + 0] - info.bbox_min[0];
+            v[k][1] = s_tri[lt * 9 + 3 * k + 1] - info.bbox_min[1];
+            v[k][2] = s_tri[lt * 9 + 3 * k + 2] - info.bbox_min[2];
+        }
+        SolidTri S;
+        solid_setup(v, info, S);
+        const int ylo = (int)S.ymin < 0 ? 0 : (int)S.ymin, zlo = (int)S.zmin < 0 ? 0 : (int)S.zmin;
+        const int yhi = S.ymax >= (float)(G - 1) ? G - 1 : (S.ymax >= 0.0f ? (int)S.ymax : -1);
+        const int zhi = S.zmax >= (float)(G - 1) ? G - 1 : (S.zmax >= 0.0f ? (int)S.zmax : -1);
+        const int ny = yhi - ylo + 1, nz = zhi - zlo + 1;
+        const int64_t cols = (int64_t)ny * nz;
+        for (int64_t c = threadIdx.x; c < cols; c += kMBlock) {
+            const int y = ylo + (int)(c / nz), z = zlo + (int)(c % nz);
+            const float py = ((float)y + 0.5f) * info.unit[1];
+            const float pz = ((float)z + 0.5f) * info.unit[2];
+            if (!column_hit(S, py, pz)) continue;
+            column_run(table, column_xmax(S, info, py, pz), y, z, (uint32_t)G, total_bits, ws, offrow);
+        }
+    }
+    if (offrow_out && offrow) atomicAdd(offrow_out, offrow);
+}
+
+// In-word suffix along x: bit b (x = 31 - b) becomes the XOR of bits 0..b.
+__device__ __forceinline__ uint32_t prefix_xor_up(uint32_t w) {
+    w ^= w << 1;
+    w ^= w << 2;
+    w ^= w << 4;
+    w ^= w << 8;
+    w ^= w << 16;
+    return w;
+}
+
+// Rows of rw = G/32 words.  rw <= 64: a wave holds 64/rw whole rows (lane =
+// word); rw > 64: a wave walks its row from the last 64-word segment down,
+// carrying the parity.
+__global__ void __launch_bounds__(256) suffix_rows_kernel(uint32_t* __restrict__ table, uint32_t G,
+                                                          uint64_t nrows) {
+    const uint32_t rw = G >> 5;
+    const int lane = __lane_id();
+    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    if (rw <= 64) {
+        const uint32_t per = 64 / rw;             // rows per wave step
+        const uint32_t r_in = (uint32_t)lane / rw, j = (uint32_t)lane % rw;
+        const bool active_lane = r_in < per;
+        for (uint64_t r0 = wave * per; r0 < nrows; r0 += nwaves * per) {
+            const uint64_t r = r0 + r_in;
+            const bool act = active_lane && r < nrows;
+            const uint64_t idx = r * rw + j;
+            const uint32_t w = act ? table[idx] : 0u;
+            const unsigned long long par = __ballot(act && (__popc(w) & 1));
+            // parity of the words with larger x in the same row: lanes j+1 .. rw-1 of this row
+            const uint32_t seg0 = r_in * rw;
+            const unsigned long long above = (active_lane && j + 1 < rw)
+                                                 ? ((par >> (seg0 + j + 1)) & ((1ull << (rw - 1 - j)) - 1ull))
+                                                 : 0ull;
+            const uint32_t carry = (uint32_t)(__popcll(above) & 1);
+            if (act) table[idx] = prefix_xor_up(w) ^ (carry ? 0xFFFFFFFFu : 0u);
+        }
+    } else {
+        for (uint64_t r = wave; r < nrows; r += nwaves) {
+            uint32_t carry = 0;  // parity of the words of larger x already done
+            for (int64_t hi = (int64_t)rw; hi > 0; hi -= 64) {
+                const int64_t lo = hi > 64 ? hi - 64 : 0;
+                const bool act = (int64_t)lane < hi - lo;
+                const uint64_t idx = r * rw + (uint64_t)lo + (uint64_t)lane;
+                const uint32_t w = act ? table[idx] : 0u;
+                const unsigned long long par = __ballot(act && (__popc(w) & 1));
+                const unsigned long long above = lane < 63 ? (par >> (lane + 1)) : 0ull;
+                const uint32_t c = (uint32_t)((__popcll(above) + carry) & 1);
+                if (act) table[idx] = prefix_xor_up(w) ^ (c ? 0xFFFFFFFFu : 0u);
+                carry = (uint32_t)((__popcll(par) + carry) & 1);
+            }
+        }
+    }
+}
+
+// Spilled run parts, XORed into the final table; one workgroup per record.
+__global__ void __launch_bounds__(256) spill_apply_kernel(uint32_t* __restrict__ table, MarkerWs ws) {
+    if (ws.misc[1] != 0u) return;  // overflowed: spill_overflow_kernel applies every spill
+    const unsigned ns = ws.misc[0] < kSpillCap ? ws.misc[0] : kSpillCap;
+    for (unsigned s = blockIdx.x; s < ns; s += gridDim.x) {
+        const uint64_t b0 = ws.spill[2 * s], b1 = ws.spill[2 * s + 1];
+        for (uint64_t w = (b0 >> 5) + threadIdx.x; w <= (b1 >> 5); w += blockDim.x) {
+            const uint32_t lo = w == (b0 >> 5) ? (uint32_t)(b0 & 31u) : 0u;
+            const uint32_t hi = w == (b1 >> 5) ? (uint32_t)(b1 & 31u) : 31u;
+            atomicXor(table + w, (0xFFFFFFFFu >> lo) & (0xFFFFFFFFu << (31u - hi)));
+        }
+    }
+}
+
+// Spill list overflow (more than kSpillCap spilled runs): every triangle's
+// spills are recomputed and applied directly.  Exits at once otherwise.
+__global__ void __launch_bounds__(256) spill_overflow_kernel(vox_info info, const float* __restrict__ tris,
+                                                             uint32_t* __restrict__ table, MarkerWs ws) {
+    if (ws.misc[1] == 0u) return;
+    const int G = (int)info.gridsize[0];
+    const uint64_t total_bits = (uint64_t)G * G * G;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < info.n_triangles;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        float v[3][3];
+        load_tri(tris, t, info, v);
+        SolidTri S;
+        solid_setup(v, info, S);
+        if (!S.valid) continue;
+        const int ylo = (int)S.ymin < 0 ? 0 : (int)S.ymin, zlo = (int)S.zmin < 0 ? 0 : (int)S.zmin;
+        const int yhi = S.ymax >= (float)(G - 1) ? G - 1 : (S.ymax >= 0.0f ? (int)S.ymax : -1);
+        const int zhi = S.zmax >= (float)(G - 1) ? G - 1 : (S.zmax >= 0.0f ? (int)S.zmax : -1);
+        for (int y = ylo; y <= yhi; y++) {
+            const float py = ((float)y + 0.5f) * info.unit[1];
+            for (int z = zlo; z <= zhi; z++) {
+                const float pz = ((float)z + 0.5f) * info.unit[2];
+                if (!column_hit(S, py, pz)) continue;
+                const int64_t xmax = column_xmax(S, info, py, pz);
+                const uint64_t row = (uint64_t)y * G + (uint64_t)z * G * G;
+                if (xmax < (int64_t)G || row + G >= total_bits) continue;
+                const uint64_t b0 = row + G;
+                const uint64_t b1 = (uint64_t)xmax < total_bits - 1 - row ? row + (uint64_t)xmax : total_bits - 1;
+                for (uint64_t w = b0 >> 5; w <= (b1 >> 5); w++) {
+                    const uint32_t lo = w == (b0 >> 5) ? (uint32_t)(b0 & 31u) : 0u;
+                    const uint32_t hi = w == (b1 >> 5) ? (uint32_t)(b1 & 31u) : 31u;
+                    atomicXor(table + w, (0xFFFFFFFFu >> lo) & (0xFFFFFFFFu << (31u - hi)));
+                }
+            }
+        }
+    }
+}
+
+// ---- host ----------------------------------------------------------------------
+static std::mutex g_mws_mutex;
+static std::map<std::pair<int, hipStream_t>, void*> g_mws;
+
+bool marker_supported(uint32_t G, int layout, bool accumulate) {
+    return layout == VOX_LAYOUT_LINEAR && (G & 31u) == 0 && !accumulate;
+}
+
+hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t* table,
+                               unsigned long long* offrow, hipStream_t stream) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    void* buf = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_mws_mutex);
+        auto it = g_mws.find({dev, stream});
+        if (it == g_mws.end()) {
+            e = hipMalloc(&buf, 256 + 16ull * kSpillCap);
+            if (e != hipSuccess) return e;
+            g_mws[{dev, stream}] = buf;
+        } else {
+            buf = it->second;
+        }
+    }
+    MarkerWs ws{(unsigned*)buf, (unsigned long long*)((char*)buf + 256)};
+    const uint32_t G = info.gridsize[0];
+    marker_init_kernel<<<1, 64, 0, stream>>>(ws.misc);
+    e = launch_fill_zero(info, table, stream);
+    if (e != hipSuccess) return e;
+    if (info.n_triangles) {
+        const uint64_t blocks = (info.n_triangles + kMBlock - 1) / kMBlock;
+        solid_marker_kernel<<<(unsigned)blocks, kMBlock, 0, stream>>>(info, tris, table, ws, offrow);
+    }
+    const uint64_t nrows = (uint64_t)G * G;
+    const uint32_t rw = G >> 5;
+    const uint64_t rows_per_wave = rw <= 64 ? 64 / rw : 1;
+    uint64_t waves = (nrows + rows_per_wave - 1) / rows_per_wave;
+    uint64_t blocks = (waves + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
+    suffix_rows_kernel<<<(unsigned)blocks, 256, 0, stream>>>(table, G, nrows);
+    spill_apply_kernel<<<64, 256, 0, stream>>>(table, ws);
+    spill_overflow_kernel<<<256, 256, 0, stream>>>(info, tris, table, ws);
+    return hipGetLastError();
+}
+
+}  // namespace voxamd

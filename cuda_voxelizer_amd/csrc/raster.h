@@ -138,19 +138,93 @@ __device__ __forceinline__ unsigned box_items(int xa, int xb, int ya, int yb, in
     return (unsigned)(yb - ya + 1) * (unsigned)(zb - za + 1) * (unsigned)nk;
 }
 
+// Mask of the passing voxels x0..x1 (inside the 32-voxel chunk at cx) of row
+// (y, z) = (py, pz) for coefficients (n, d1, d2, xy[9], zx[9]); the per-voxel
+// arithmetic of vox_device.h's plane_pass / edges_pass with the row-invariant
+// products hoisted (each product and sum is the same binary32 operation).
+__device__ __forceinline__ uint32_t row_mask(const float* n, float d1, float d2, const float* xy, const float* zx,
+                                             float ux, float py, float pz, int x0, int x1, int cx) {
+    const float nypy = n[1] * py, nzpz = n[2] * pz;
+    float bxy[3], azx[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        bxy[i] = xy[3 + i] * py;
+        azx[i] = zx[i] * pz;
+    }
+    uint32_t mask = 0;
+    for (int x = x0; x <= x1; x++) {
+        const float px = (float)x * ux;
+        float t = n[0] * px;
+        t = t + nypy;
+        t = t + nzpz;
+        const float r = (t + d1) * (t + d2);
+        bool ok = !(r > 0.0f);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            float a = xy[i] * px;
+            a = a + bxy[i];
+            a = a + xy[6 + i];
+            float b = azx[i];
+            b = b + zx[3 + i] * px;
+            b = b + zx[6 + i];
+            ok = ok && !(a < 0.0f) && !(b < 0.0f);
+        }
+        if (ok) mask |= 1u << (x - cx);
+    }
+    return mask;
+}
+
+// Boxes this small are rasterized by their own lane (no LDS round trip, no
+// work-item search); larger ones are shared over the wave.
+constexpr int kLaneX = 32;      // x extent (at most two 32-voxel chunks)
+constexpr int kLaneRows = 16;   // (y, z) rows
+constexpr int kLaneVox = 64;    // voxels
+
+// Wave-level barrier: the wave's LDS writes are visible to all its lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Rasterize the wave's triangles.  `have` says whether this lane brings a
 // triangle; (xa..zb) is its (clipped) voxel box.  emit(y, z, cx, mask) gets
 // bit k of mask set for each passing voxel cx + k (k < 32, cx a multiple of
-// 32).  W is this wave's LDS area.  Every wave of the workgroup must call it
-// the same number of times (it synchronises the workgroup).
+// 32).  W is this wave's LDS area (synchronised at wave level only).
 template <class Emit>
 __device__ __forceinline__ void wave_raster(WaveRaster& W, const vox_info& info, bool have, const SurfTri& T,
                                             int xa, int xb, int ya, int yb, int za, int zb, Emit&& emit) {
     const int lane = __lane_id();
+    const float ux = info.unit[0], uy = info.unit[1], uz = info.unit[2];
+    const bool empty = !have || xa > xb || ya > yb || za > zb;
+    const int nx = xb - xa + 1, nrows = (yb - ya + 1) * (zb - za + 1);
+    const bool small = !empty && nx <= kLaneX && nrows <= kLaneRows && nx * nrows <= kLaneVox;
+    if (small) {
+        const float xy[9] = {T.xy.a[0], T.xy.a[1], T.xy.a[2], T.xy.b[0], T.xy.b[1], T.xy.b[2],
+                             T.xy.d[0], T.xy.d[1], T.xy.d[2]};
+        const float zx[9] = {T.zx.a[0], T.zx.a[1], T.zx.a[2], T.zx.b[0], T.zx.b[1], T.zx.b[2],
+                             T.zx.d[0], T.zx.d[1], T.zx.d[2]};
+        const int c0 = xa & ~31, c1 = xb & ~31;
+        for (int z = za; z <= zb; z++) {
+            const float pz = (float)z * uz;
+            for (int y = ya; y <= yb; y++) {
+                const float py = (float)y * uy;
+                if (!edges_pass(T.yz, py, pz)) continue;
+                const uint32_t m0 = row_mask(T.n, T.d1, T.d2, xy, zx, ux, py, pz, xa, c1 > c0 ? c0 + 31 : xb, c0);
+                if (m0) emit(y, z, c0, m0);
+                if (c1 > c0) {
+                    const uint32_t m1 = row_mask(T.n, T.d1, T.d2, xy, zx, ux, py, pz, c1, xb, c1);
+                    if (m1) emit(y, z, c1, m1);
+                }
+            }
+        }
+    }
+    const bool big = !empty && !small;
+    if (!__ballot(big)) return;  // uniform over the wave
     int nk = 1, band = 0;
     float sx = 0.0f;
-    if (have && xa <= xb) band_chunks(T, info, xa, xb, yb, zb, nk, band, sx);
-    unsigned items = have ? box_items(xa, xb, ya, yb, za, zb, nk) : 0u;
+    if (big) band_chunks(T, info, xa, xb, yb, zb, nk, band, sx);
+    unsigned items = big ? box_items(xa, xb, ya, yb, za, zb, nk) : 0u;
     if (items) publish(W.rec[lane], T, xa, xb, ya, yb, za, zb, nk, band, sx);
     // inclusive scan of item counts over the wave
     unsigned incl = items;
@@ -161,8 +235,7 @@ __device__ __forceinline__ void wave_raster(WaveRaster& W, const vox_info& info,
     }
     W.incl[lane] = incl;
     const unsigned total = __shfl(incl, 63);
-    __syncthreads();
-    const float ux = info.unit[0], uy = info.unit[1], uz = info.unit[2];
+    wave_sync();
     for (unsigned j = lane; j < total; j += 64) {
         // owner: first lane whose inclusive prefix exceeds j
         int lo = 0, hi = 63;
@@ -172,9 +245,9 @@ __device__ __forceinline__ void wave_raster(WaveRaster& W, const vox_info& info,
         }
         const SurfRec& R = W.rec[lo];
         const unsigned q = j - (lo ? W.incl[lo - 1] : 0u);
-        const unsigned nk = (unsigned)R.nk;
+        const unsigned nkr = (unsigned)R.nk;
         const unsigned ny = (unsigned)(R.yb - R.ya + 1);
-        const unsigned row = q / nk, k = q - row * nk;
+        const unsigned row = q / nkr, k = q - row * nkr;
         const int y = R.ya + (int)(row % ny), z = R.za + (int)(row / ny);
         const float py = (float)y * uy, pz = (float)z * uz;
         int bx0 = R.xa, bx1 = R.xb;
@@ -186,37 +259,10 @@ __device__ __forceinline__ void wave_raster(WaveRaster& W, const vox_info& info,
         if (cx > bx1) continue;
         if (!pass3(R.yz, py, pz)) continue;
         const int x0 = cx > bx0 ? cx : bx0, x1 = (cx + 31) < bx1 ? (cx + 31) : bx1;
-        // row-invariant parts of the plane, XY and ZX tests
-        const float nypy = R.n[1] * py, nzpz = R.n[2] * pz;
-        float bxy[3], azx[3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            bxy[i] = R.xy[3 + i] * py;
-            azx[i] = R.zx[i] * pz;
-        }
-        uint32_t mask = 0;
-        for (int x = x0; x <= x1; x++) {
-            const float px = (float)x * ux;
-            float t = R.n[0] * px;
-            t = t + nypy;
-            t = t + nzpz;
-            const float r = (t + R.d1) * (t + R.d2);
-            bool ok = !(r > 0.0f);
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                float a = R.xy[i] * px;
-                a = a + bxy[i];
-                a = a + R.xy[6 + i];
-                float b = azx[i];
-                b = b + R.zx[3 + i] * px;
-                b = b + R.zx[6 + i];
-                ok = ok && !(a < 0.0f) && !(b < 0.0f);
-            }
-            if (ok) mask |= 1u << (x - cx);
-        }
+        const uint32_t mask = row_mask(R.n, R.d1, R.d2, R.xy, R.zx, ux, py, pz, x0, x1, cx);
         if (mask) emit(y, z, cx, mask);
     }
-    __syncthreads();
+    wave_sync();  // W is reused by the wave's next call
 }
 
 // Split a 32-voxel x-chunk mask (bit k = voxel cx + k, cx a multiple of 32) of

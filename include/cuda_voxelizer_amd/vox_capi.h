@@ -59,7 +59,7 @@ typedef enum vox_status {
                                     it the call defines every word of the table
                                     (the zero-fill is fused into the kernels). */
 /* Algorithm selection (at most one; default: the library's choice per grid,
-   layout and mode -- chunk for surface, tile for solid).  All algorithms give
+   layout and mode -- chunk for surface, marker (else tile) for solid).  All algorithms give
    identical tables. */
 #define VOX_FLAG_ALGO_DIRECT 0x2u /* one triangle per lane, global atomics per word */
 #define VOX_FLAG_ALGO_TILE 0x4u   /* triangles binned to output tiles; one workgroup
@@ -67,6 +67,10 @@ typedef enum vox_status {
 #define VOX_FLAG_ALGO_CHUNK 0x8u  /* surface only: one workgroup per chunk of
                                     consecutive triangles merges words in an LDS
                                     hash, then one atomicOr per distinct word */
+#define VOX_FLAG_ALGO_MARKER 0x10u /* solid only (linear layout, G % 32 == 0, no
+                                    accumulate; otherwise tile): one marker bit per
+                                    hit column at the run end, then every row is
+                                    replaced by its suffix XOR along x */
 
 /* ---- setup / diagnostics ---------------------------------------------------- */
 vox_status vox_init(int device);            /* select + probe a device (initCuda) */

@@ -15,7 +15,7 @@ from tests.conftest import GOLDEN
 pytestmark = pytest.mark.gpu
 
 ALGOS = ["auto", "tile", "chunk", "direct"]
-SOLID_ALGOS = ["auto", "tile", "direct"]
+SOLID_ALGOS = ["auto", "marker", "tile", "direct"]
 
 
 def _dev_tris(tris, dev):
@@ -205,3 +205,22 @@ def test_sh_sphere_1024_solid_full(oracle, gpu, algo):
     # column densely, and the fill is a proper subset of the grid
     bits = int(np.unpackbits(got.view(np.uint8)).sum())
     assert 0.2 * 1024 ** 3 < bits < 0.6 * 1024 ** 3
+
+
+@pytest.mark.parametrize("grid", [32, 96, 2112])
+def test_solid_row_widths(oracle, gpu, grid):
+    """Rows of 1, 3 and 66 words: the marker algorithm's suffix pass packs
+    several rows per wave (G <= 2048) or walks 64-word segments with a carry
+    (66 = one full segment + a partial one).  A small closed mesh inside a
+    large box keeps the oracle's per-bit runs short."""
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import meshgen
+    v, f = meshgen.icosphere_sh(level=3, seed=grid)
+    radius = 0.04 if grid > 1000 else 0.3  # >= 5 voxels
+    v = (v * np.float32(radius) + np.float32(0.3)).astype(np.float32)
+    box = V.AABox(np.array([-1, -1, -1], np.float32), np.array([1, 1, 1], np.float32))
+    info = V.VoxInfo(box, grid, f.shape[0])
+    print(f"solid row widths G={grid}", flush=True)
+    got, exp = _run(oracle, gpu, meshgen.to_aos(v, f), grid, False, True, "marker", info=info)
+    assert _mismatch(got, exp) == 0
+    assert np.unpackbits(got.view(np.uint8)).any()
