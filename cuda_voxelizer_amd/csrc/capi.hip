@@ -140,7 +140,7 @@ vox_status run(const vox_info* info, const float* d_tris, uint32_t* d_table, int
     else algo = chunk_supported(G) ? CHUNK : (have_plan ? TILE : DIRECT);
     hipError_t e;
     if (algo == MARKER) {
-        e = launch_solid_marker(*info, d_tris, d_table, (unsigned long long*)d_offrow, st);
+        e = launch_solid_marker(*info, d_tris, d_table, layout, accumulate, (unsigned long long*)d_offrow, st);
         if (e != hipSuccess) return hip_fail(e, "marker kernels launch");
         return VOX_OK;
     }

@@ -6,20 +6,25 @@
 // of its column with xmax >= x, so it is enough to toggle ONE marker bit at
 // (xmax, y, z) per run and then replace every row by its suffix XOR along x:
 //
-//   1. fill      zero the table (fill_zero_kernel)
+//   1. fill      zero the marker table: the output table itself (linear
+//                layout, no accumulate) or a linear scratch table
 //   2. markers   per triangle (one lane each; triangles with many columns
 //                are split over the workgroup): solid setup, column tests,
 //                xmax, one non-returning atomicXor per hit column
 //   3. suffix    per row (G/32 words, lanes = words): in-word prefix XOR from
 //                bit 0 up (larger x = lower bit) plus the parity of the words
-//                of larger x (a ballot), in place
+//                of larger x (a ballot).  In place for the linear layout;
+//                from the scratch table, one workgroup per block of 32 x rz
+//                rows staged in LDS, for the Morton layout (4x4x2 brick words
+//                assembled from 8 rows at a time) and for accumulation
+//                (XOR into the caller's table)
 //   4. spills    runs with xmax >= G continue along the linear index into the
 //                following rows (as the reference indexes them): their
 //                in-row part is a marker at G-1, the rest is recorded and
 //                XORed into the final table afterwards (word masks, atomics)
 //
-// Linear layout with G % 32 == 0 (rows are whole words), table defined by the
-// call (no accumulate).  Per-column arithmetic: vox_device.h.
+// G % 32 == 0 (rows are whole words); Morton: power-of-two G up to 4096.
+// Per-column arithmetic: vox_device.h (the reference CPU path bit for bit).
 #include "vox_device.h"
 #include "vox_host.h"
 #include "vox_kernels.h"
@@ -49,14 +54,15 @@ __global__ void __launch_bounds__(64) marker_init_kernel(unsigned* misc) {
 
 // One hit column: marker toggle (and spill record) of the run x = 0..xmax.
 __device__ __forceinline__ void column_run(uint32_t* __restrict__ table, int64_t xmax, int y, int z, uint32_t G,
-                                           uint64_t total_bits, const MarkerWs& ws, unsigned long long& offrow) {
+                                           uint64_t total_bits, const MarkerWs& ws, unsigned long long& offrow,
+                                           bool spills) {
     if (xmax < 0 || xmax >= (int64_t)G) offrow++;
     if (xmax < 0) return;
     const uint64_t row = (uint64_t)y * G + (uint64_t)z * G * G;
     const uint32_t xm = xmax < (int64_t)G ? (uint32_t)xmax : G - 1;
     const uint64_t idx = row + xm;
     atomicXor(table + (idx >> 5), bit_mask(idx));
-    if (xmax >= (int64_t)G && row + G < total_bits) {
+    if (spills && xmax >= (int64_t)G && row + G < total_bits) {
         // bits row+G .. row+xmax, clipped to the table
         const uint64_t last = (uint64_t)xmax < total_bits - 1 - row ? row + (uint64_t)xmax : total_bits - 1;
         const unsigned slot = atomicAdd(ws.misc, 1u);
@@ -71,7 +77,8 @@ __device__ __forceinline__ void column_run(uint32_t* __restrict__ table, int64_t
 
 __global__ void __launch_bounds__(kMBlock) solid_marker_kernel(vox_info info, const float* __restrict__ tris,
                                                                 uint32_t* __restrict__ table, MarkerWs ws,
-                                                                unsigned long long* __restrict__ offrow_out) {
+                                                                unsigned long long* __restrict__ offrow_out,
+                                                                int spills) {
     __shared__ float s_tri[kMBlock * 9];
     __shared__ unsigned queue[kMQueue];
     __shared__ unsigned qn;
@@ -120,7 +127,7 @@ __global__ void __launch_bounds__(kMBlock) solid_marker_kernel(vox_info info, co
                     for (int z = zlo; z <= zhi; z++) {
                         const float pz = ((float)z + 0.5f) * info.unit[2];
                         if (!column_hit(S, py, pz)) continue;
-                        column_run(table, column_xmax(S, info, py, pz), y, z, (uint32_t)G, total_bits, ws, offrow);
+                        column_run(table, column_xmax(S, info, py, pz), y, z, (uint32_t)G, total_bits, ws, offrow, spills != 0);
                     }
                 }
             }
@@ -149,7 +156,7 @@ __global__ void __launch_bounds__(kMBlock) solid_marker_kernel(vox_info info, co
             const float py = ((float)y + 0.5f) * info.unit[1];
             const float pz = ((float)z + 0.5f) * info.unit[2];
             if (!column_hit(S, py, pz)) continue;
-            column_run(table, column_xmax(S, info, py, pz), y, z, (uint32_t)G, total_bits, ws, offrow);
+            column_run(table, column_xmax(S, info, py, pz), y, z, (uint32_t)G, total_bits, ws, offrow, spills != 0);
         }
     }
     if (offrow_out && offrow) atomicAdd(offrow_out, offrow);
@@ -210,6 +217,105 @@ __global__ void __launch_bounds__(256) suffix_rows_kernel(uint32_t* __restrict__
     }
 }
 
+// Rows of the linear marker table -> output table, one workgroup per block
+// of 32 rows in y x rz rows in z (all x), staged in LDS.  Linear output: the
+// final words are XORed into (accumulate) or stored to the block's
+// contiguous rows.  Morton output: a word is a 4x4x2 brick (code bits 0..4 =
+// x0 y0 z0 x1 y1); a thread takes one linear word (32 x) of the 8 rows of a
+// brick row and emits the 8 Morton words of its 8 groups of 4 x.
+__global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __restrict__ markers,
+                                                           uint32_t* __restrict__ table, uint32_t G, uint32_t rz,
+                                                           int layout, int accumulate) {
+    extern __shared__ uint32_t tile[];  // [rz * 32 rows][rw]
+    const uint32_t rw = G >> 5, nby = G >> 5;
+    const uint32_t rb = blockIdx.x;
+    const uint32_t nrows = 32u * rz;
+    const uint32_t z0 = (rb / nby) * rz, y0 = (rb % nby) * 32u;
+    // rows (y0.., z) for each z of the block are 32 * rw contiguous words
+    for (uint32_t i = threadIdx.x; i < nrows * rw; i += 256) {
+        const uint32_t zl = i / (32u * rw), k = i % (32u * rw);
+        tile[i] = markers[((uint64_t)(z0 + zl) * G + y0) * rw + k];
+    }
+    __syncthreads();
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    const bool linear = layout == VOX_LAYOUT_LINEAR;
+    if (rw <= 64) {
+        const uint32_t per = 64 / rw;
+        const uint32_t r_in = (uint32_t)lane / rw, j = (uint32_t)lane % rw;
+        const bool active_lane = r_in < per;
+        for (uint32_t r0 = (uint32_t)wave * per; r0 < nrows; r0 += 4 * per) {
+            const uint32_t r = r0 + r_in;
+            const bool act = active_lane && r < nrows;
+            const uint32_t w = act ? tile[r * rw + j] : 0u;
+            const unsigned long long par = __ballot(act && (__popc(w) & 1));
+            const uint32_t seg0 = r_in * rw;
+            const unsigned long long above =
+                (active_lane && j + 1 < rw) ? ((par >> (seg0 + j + 1)) & ((1ull << (rw - 1 - j)) - 1ull)) : 0ull;
+            const uint32_t fin = prefix_xor_up(w) ^ ((__popcll(above) & 1) ? 0xFFFFFFFFu : 0u);
+            if (act) {
+                if (linear) {
+                    const uint64_t idx = ((uint64_t)(z0 + r / 32u) * G + y0 + (r & 31u)) * rw + j;
+                    table[idx] = accumulate ? (table[idx] ^ fin) : fin;
+                } else {
+                    tile[r * rw + j] = fin;
+                }
+            }
+        }
+    } else {
+        for (uint32_t r = (uint32_t)wave; r < nrows; r += 4) {
+            uint32_t carry = 0;
+            for (int64_t hi = (int64_t)rw; hi > 0; hi -= 64) {
+                const int64_t lo = hi > 64 ? hi - 64 : 0;
+                const bool act = (int64_t)lane < hi - lo;
+                const uint32_t j = (uint32_t)lo + (uint32_t)lane;
+                const uint32_t w = act ? tile[r * rw + j] : 0u;
+                const unsigned long long par = __ballot(act && (__popc(w) & 1));
+                const unsigned long long above = lane < 63 ? (par >> (lane + 1)) : 0ull;
+                const uint32_t fin = prefix_xor_up(w) ^ (((__popcll(above) + carry) & 1) ? 0xFFFFFFFFu : 0u);
+                carry = (uint32_t)((__popcll(par) + carry) & 1);
+                if (act) {
+                    if (linear) {
+                        const uint64_t idx = ((uint64_t)(z0 + r / 32u) * G + y0 + (r & 31u)) * rw + j;
+                        table[idx] = accumulate ? (table[idx] ^ fin) : fin;
+                    } else {
+                        tile[r * rw + j] = fin;
+                    }
+                }
+            }
+        }
+    }
+    if (linear) return;
+    __syncthreads();
+    const uint32_t groups = 8u * (rz / 2u);  // brick rows (4 y x 2 z) in the block
+    for (uint32_t t = threadIdx.x; t < groups * rw; t += 256) {
+        const uint32_t j = t % rw, br = t / rw;
+        const uint32_t yb = br % 8u, zb = br / 8u;
+        uint32_t row_w[8];  // row q = (y1 << 2 | z0 << 1 | y0) -> its linear word j
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const uint32_t y0b = (uint32_t)q & 1u, z0b = ((uint32_t)q >> 1) & 1u, y1b = ((uint32_t)q >> 2) & 1u;
+            row_w[q] = tile[((zb * 2u + z0b) * 32u + yb * 4u + y0b + 2u * y1b) * rw + j];
+        }
+#pragma unroll
+        for (int xg = 0; xg < 8; xg++) {
+            uint32_t word = 0;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const uint32_t y0b = (uint32_t)q & 1u, z0b = ((uint32_t)q >> 1) & 1u, y1b = ((uint32_t)q >> 2) & 1u;
+                // nibble bits 3..0 = x offsets 0..3; offset o = x0 + 2 x1 goes to code bit c0 | x0 | x1 << 3,
+                // i.e. word bit 31 - c0 - x0 - 8 x1: offsets 0, 1 are the adjacent word bits 31-c0, 30-c0
+                // and offsets 2, 3 the adjacent bits 23-c0, 22-c0 (c0 is a compile-time constant here)
+                const uint32_t nib = (row_w[q] >> (28 - 4 * xg)) & 0xFu;
+                const uint32_t c0 = (y0b << 1) | (z0b << 2) | (y1b << 4);
+                word |= ((nib >> 2) << (30u - c0)) | ((nib & 3u) << (22u - c0));
+            }
+            const uint64_t code = morton3(j * 32u + (uint32_t)xg * 4u, y0 + yb * 4u, z0 + zb * 2u);
+            uint32_t* dst = table + (code >> 5);
+            *dst = accumulate ? (*dst ^ word) : word;
+        }
+    }
+}
+
 // Spilled run parts, XORed into the final table; one workgroup per record.
 __global__ void __launch_bounds__(256) spill_apply_kernel(uint32_t* __restrict__ table, MarkerWs ws) {
     if (ws.misc[1] != 0u) return;  // overflowed: spill_overflow_kernel applies every spill
@@ -262,48 +368,80 @@ __global__ void __launch_bounds__(256) spill_overflow_kernel(vox_info info, cons
 }
 
 // ---- host ----------------------------------------------------------------------
+struct MarkerBuffers {
+    void* ws = nullptr;       // misc + spill records
+    void* scratch = nullptr;  // linear marker table (Morton output, accumulation)
+    uint64_t scratch_bytes = 0;
+};
 static std::mutex g_mws_mutex;
-static std::map<std::pair<int, hipStream_t>, void*> g_mws;
+static std::map<std::pair<int, hipStream_t>, MarkerBuffers> g_mws;
 
 bool marker_supported(uint32_t G, int layout, bool accumulate) {
-    return layout == VOX_LAYOUT_LINEAR && (G & 31u) == 0 && !accumulate;
+    (void)accumulate;
+    if (G < 32 || (G & 31u)) return false;
+    if (layout == VOX_LAYOUT_MORTON) return (G & (G - 1)) == 0 && G <= 4096;
+    return true;
 }
 
-hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t* table,
-                               unsigned long long* offrow, hipStream_t stream) {
+hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t* table, int layout,
+                               bool accumulate, unsigned long long* offrow, hipStream_t stream) {
+    const uint32_t G = info.gridsize[0];
+    const bool in_place = layout == VOX_LAYOUT_LINEAR && !accumulate;
+    const uint64_t table_bytes = (uint64_t)G * G * G / 8;
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
-    void* buf = nullptr;
+    MarkerWs ws;
+    uint32_t* markers = table;
     {
         std::lock_guard<std::mutex> lock(g_mws_mutex);
-        auto it = g_mws.find({dev, stream});
-        if (it == g_mws.end()) {
-            e = hipMalloc(&buf, 256 + 16ull * kSpillCap);
+        MarkerBuffers& b = g_mws[{dev, stream}];
+        if (!b.ws) {
+            e = hipMalloc(&b.ws, 256 + 16ull * kSpillCap);
             if (e != hipSuccess) return e;
-            g_mws[{dev, stream}] = buf;
-        } else {
-            buf = it->second;
         }
+        if (!in_place && b.scratch_bytes < table_bytes) {
+            if (b.scratch) {
+                e = hipStreamSynchronize(stream);  // a previous call on this stream may still read it
+                if (e == hipSuccess) e = hipFree(b.scratch);
+                b.scratch = nullptr;
+                b.scratch_bytes = 0;
+                if (e != hipSuccess) return e;
+            }
+            e = hipMalloc(&b.scratch, table_bytes);
+            if (e != hipSuccess) return e;
+            b.scratch_bytes = table_bytes;
+        }
+        ws = MarkerWs{(unsigned*)b.ws, (unsigned long long*)((char*)b.ws + 256)};
+        if (!in_place) markers = (uint32_t*)b.scratch;
     }
-    MarkerWs ws{(unsigned*)buf, (unsigned long long*)((char*)buf + 256)};
-    const uint32_t G = info.gridsize[0];
     marker_init_kernel<<<1, 64, 0, stream>>>(ws.misc);
-    e = launch_fill_zero(info, table, stream);
+    e = launch_fill_zero(info, markers, stream);
     if (e != hipSuccess) return e;
+    const bool linear = layout == VOX_LAYOUT_LINEAR;
     if (info.n_triangles) {
         const uint64_t blocks = (info.n_triangles + kMBlock - 1) / kMBlock;
-        solid_marker_kernel<<<(unsigned)blocks, kMBlock, 0, stream>>>(info, tris, table, ws, offrow);
+        solid_marker_kernel<<<(unsigned)blocks, kMBlock, 0, stream>>>(info, tris, markers, ws, offrow,
+                                                                       linear ? 1 : 0);
     }
-    const uint64_t nrows = (uint64_t)G * G;
-    const uint32_t rw = G >> 5;
-    const uint64_t rows_per_wave = rw <= 64 ? 64 / rw : 1;
-    uint64_t waves = (nrows + rows_per_wave - 1) / rows_per_wave;
-    uint64_t blocks = (waves + 3) / 4;
-    if (blocks > 65536) blocks = 65536;
-    suffix_rows_kernel<<<(unsigned)blocks, 256, 0, stream>>>(table, G, nrows);
-    spill_apply_kernel<<<64, 256, 0, stream>>>(table, ws);
-    spill_overflow_kernel<<<256, 256, 0, stream>>>(info, tris, table, ws);
+    if (in_place) {
+        const uint64_t nrows = (uint64_t)G * G;
+        const uint32_t rw = G >> 5;
+        const uint64_t rows_per_wave = rw <= 64 ? 64 / rw : 1;
+        const uint64_t waves = (nrows + rows_per_wave - 1) / rows_per_wave;
+        uint64_t blocks = (waves + 3) / 4;
+        if (blocks > 65536) blocks = 65536;
+        suffix_rows_kernel<<<(unsigned)blocks, 256, 0, stream>>>(table, G, nrows);
+    } else {
+        const uint32_t rz = linear ? 1u : (G <= 1024u ? 8u : 2u);
+        const uint32_t nblocks = (G / 32u) * (G / rz);
+        marker_build_kernel<<<nblocks, 256, 4ull * 32u * rz * (G / 32u), stream>>>(markers, table, G, rz, layout,
+                                                                                 accumulate ? 1 : 0);
+    }
+    if (linear) {
+        spill_apply_kernel<<<64, 256, 0, stream>>>(table, ws);
+        spill_overflow_kernel<<<256, 256, 0, stream>>>(info, tris, table, ws);
+    }
     return hipGetLastError();
 }
 

@@ -636,38 +636,6 @@ __global__ void __launch_bounds__(kBlock) wide_surface_kernel(ItemDev one, ItemM
     }
 }
 
-__device__ __forceinline__ void xor_linear_run(uint32_t* table, uint64_t b0, uint64_t b1) {
-    for (uint64_t w = b0 >> 5; w <= (b1 >> 5); w++) {
-        const uint32_t s = (w == (b0 >> 5)) ? (uint32_t)(b0 & 31u) : 0u;
-        const uint32_t e = (w == (b1 >> 5)) ? (uint32_t)(b1 & 31u) : 31u;
-        atomicXor(table + w, (0xFFFFFFFFu >> s) & (0xFFFFFFFFu << (31u - e)));
-    }
-}
-
-// Full XOR run of one hit column (wide triangles), reference indexing.
-__device__ __forceinline__ void xor_column_run(uint32_t* table, int layout, int G, int y, int z, int64_t xmax) {
-    if (layout == VOX_LAYOUT_LINEAR) {
-        const uint64_t total = (uint64_t)G * (uint64_t)G * (uint64_t)G;
-        const uint64_t b0 = ((uint64_t)y + (uint64_t)z * (uint64_t)G) * (uint64_t)G;
-        const uint64_t room = total - 1 - b0;
-        xor_linear_run(table, b0, b0 + ((uint64_t)xmax < room ? (uint64_t)xmax : room));
-    } else {
-        const int xe = xmax < G ? (int)xmax : G - 1;
-        uint64_t cur = ~0ull;
-        uint32_t mask = 0;
-        for (int x = 0; x <= xe; x++) {
-            const uint64_t idx = morton3((uint32_t)x, (uint32_t)y, (uint32_t)z);
-            if ((idx >> 5) != cur) {
-                if (mask) atomicXor(table + cur, mask);
-                cur = idx >> 5;
-                mask = 0;
-            }
-            mask |= bit_mask(idx);
-        }
-        if (mask) atomicXor(table + cur, mask);
-    }
-}
-
 __global__ void __launch_bounds__(kBlock) wide_solid_kernel(ItemDev one, ItemMap M,
                                                            const unsigned* __restrict__ wide_ids,
                                                            const unsigned* __restrict__ misc,

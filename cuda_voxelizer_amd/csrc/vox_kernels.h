@@ -26,8 +26,8 @@ hipError_t launch_tiled(const vox_info& info, const float* tris, uint32_t* table
                         unsigned long long* offrow, hipStream_t stream);
 
 bool marker_supported(uint32_t G, int layout, bool accumulate);
-hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t* table,
-                               unsigned long long* offrow, hipStream_t stream);
+hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t* table, int layout,
+                               bool accumulate, unsigned long long* offrow, hipStream_t stream);
 
 bool chunk_supported(uint32_t G);
 hipError_t launch_fill_zero(const vox_info& info, uint32_t* table, hipStream_t stream);

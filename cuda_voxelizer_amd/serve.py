@@ -116,6 +116,7 @@ class VoxServer:
         self.admitted: collections.deque[_Admitted] = collections.deque()
         self.inflight: list[_InFlight] = []
         self.meshes: collections.OrderedDict = collections.OrderedDict()  # key -> (tensor, nbytes, users)
+        self.boxes: dict = {}  # key -> cube bounding box (createMeshBBCube of the mesh bbox)
         self.used = 0
         self.rejected: list[int] = []
         self.stats = {"batches": 0, "jobs": 0, "mesh_uploads": 0, "mesh_hits": 0}
@@ -189,8 +190,15 @@ class VoxServer:
                 del self.meshes[key]
                 self.used -= nb
 
+    def _info(self, req: Request):
+        """voxinfo of a request; the mesh's cube box is computed once per mesh."""
+        from . import VoxInfo, create_mesh_bbox_cube, mesh_bbox
+        box = self.boxes.get(req.mesh_key)
+        if box is None:
+            box = self.boxes[req.mesh_key] = create_mesh_bbox_cube(mesh_bbox(req.tris.reshape(-1, 3)))
+        return VoxInfo(box, (req.grid, req.grid, req.grid), req.tris.shape[0])
+
     def _admit(self) -> None:
-        from . import voxinfo_for_mesh
         torch = self.torch
         while self.queue:
             req = self.queue[0]
@@ -204,7 +212,7 @@ class VoxServer:
                 if self.used + need > self.budget:
                     return  # wait for tables to be released / batches to finish
             self.queue.popleft()
-            info = voxinfo_for_mesh(req.tris.reshape(-1, 3), req.grid, req.tris.shape[0])
+            info = self._info(req)
             fresh = req.mesh_key not in self.meshes
             tris_dev = self._mesh(req)
             table = torch.empty(self._table_bytes(req.grid) // 4, dtype=torch.int32, device=self.device)
@@ -277,16 +285,19 @@ class VoxServer:
 
 
 def serve_plain(requests: list[Request], device: str = "cuda:0", clock=time.perf_counter,
-                realtime: bool = True, mesh_cache: dict | None = None) -> list[Result]:
+                realtime: bool = True, mesh_cache: dict | None = None,
+                box_cache: dict | None = None) -> list[Result]:
     """One plain voxelization call per request, in arrival order (FIFO).
 
     With ``realtime`` the loop waits for each request's arrival time (relative
-    to the start), so latencies include queueing behind earlier requests."""
+    to the start), so latencies include queueing behind earlier requests.
+    Meshes (and their cube boxes) are cached per mesh key, as in the server."""
     import torch
 
-    from . import voxelize_solid_table, voxelize_surface, voxinfo_for_mesh
+    from . import VoxInfo, create_mesh_bbox_cube, mesh_bbox, voxelize_solid_table, voxelize_surface
     dev = torch.device(device)
     cache: dict = {} if mesh_cache is None else mesh_cache
+    boxes: dict = {} if box_cache is None else box_cache
     out: list[Result] = []
     t0 = clock()
     for req in sorted(requests, key=lambda r: r.arrival):
@@ -297,7 +308,10 @@ def serve_plain(requests: list[Request], device: str = "cuda:0", clock=time.perf
         if req.mesh_key not in cache:
             cache[req.mesh_key] = torch.from_numpy(
                 np.ascontiguousarray(req.tris, np.float32).reshape(-1)).to(dev)
-        info = voxinfo_for_mesh(req.tris.reshape(-1, 3), req.grid, req.tris.shape[0])
+        box = boxes.get(req.mesh_key)
+        if box is None:
+            box = boxes[req.mesh_key] = create_mesh_bbox_cube(mesh_bbox(req.tris.reshape(-1, 3)))
+        info = VoxInfo(box, (req.grid, req.grid, req.grid), req.tris.shape[0])
         admitted = clock()
         if req.solid:
             table = voxelize_solid_table(info, cache[req.mesh_key], morton=req.morton)

@@ -67,10 +67,12 @@ typedef enum vox_status {
 #define VOX_FLAG_ALGO_CHUNK 0x8u  /* surface only: one workgroup per chunk of
                                     consecutive triangles merges words in an LDS
                                     hash, then one atomicOr per distinct word */
-#define VOX_FLAG_ALGO_MARKER 0x10u /* solid only (linear layout, G % 32 == 0, no
-                                    accumulate; otherwise tile): one marker bit per
-                                    hit column at the run end, then every row is
-                                    replaced by its suffix XOR along x */
+#define VOX_FLAG_ALGO_MARKER 0x10u /* solid only (G % 32 == 0; Morton: power of two
+                                    up to 4096; otherwise tile): one marker per hit
+                                    column at the run end, gathered in lists per
+                                    block of rows; each block of rows is built in
+                                    LDS as the suffix XOR of its markers along x
+                                    and written once */
 
 /* ---- setup / diagnostics ---------------------------------------------------- */
 vox_status vox_init(int device);            /* select + probe a device (initCuda) */

@@ -224,3 +224,46 @@ def test_solid_row_widths(oracle, gpu, grid):
     got, exp = _run(oracle, gpu, meshgen.to_aos(v, f), grid, False, True, "marker", info=info)
     assert _mismatch(got, exp) == 0
     assert np.unpackbits(got.view(np.uint8)).any()
+
+
+@pytest.mark.parametrize("algo", SOLID_ALGOS)
+@pytest.mark.parametrize("morton", [False, True])
+def test_solid_runs_leaving_the_row_gpu(oracle, gpu, algo, morton):
+    """Triangles in planes x = c beyond the grid (runs spill along the linear
+    index into the next rows; Morton runs stop at G-1), inside it, and behind
+    it (empty runs), with few columns (listed) and many (wide), including the
+    last rows of the table (spill clipped at the table end)."""
+    import cuda_voxelizer_amd as V
+    grid = 32
+    box = V.AABox(np.zeros(3, np.float32), np.full(3, 32, np.float32))
+    yzs = [np.array([[0.2, 0.3], [3.1, 0.6], [0.9, 3.4]], np.float32),        # a few columns
+           np.array([[0.2, 0.3], [31.1, 0.6], [0.9, 31.4]], np.float32),      # hundreds: wide
+           np.array([[29.2, 29.3], [31.9, 29.6], [29.9, 31.9]], np.float32)]  # the last rows
+    tris = []
+    for c in (33.7, 70.2, 0.2, 12.6, -0.7):
+        for yz in yzs:
+            tris.append(np.concatenate([np.full((3, 1), c, np.float32), yz], 1))
+    tris = np.stack(tris).astype(np.float32)
+    info = V.VoxInfo(box, grid, tris.shape[0])
+    got, exp = _run(oracle, gpu, tris, grid, morton, True, algo, info=info)
+    assert _mismatch(got, exp) == 0
+
+
+@pytest.mark.parametrize("algo", ["marker", "tile"])
+@pytest.mark.parametrize("grid,level", [(64, 1), (128, 2), (256, 5)])
+def test_solid_coarse_meshes_both_layouts(oracle, gpu, algo, grid, level):
+    """Coarse meshes (large triangles: the marker algorithm's wide path) and a
+    fine one, linear and Morton."""
+    from cuda_voxelizer_amd import meshgen
+    v, f = meshgen.icosphere_sh(level=level, seed=grid)
+    for morton in (False, True):
+        got, exp = _run(oracle, gpu, meshgen.to_aos(v, f), grid, morton, True, algo)
+        assert _mismatch(got, exp) == 0, morton
+
+
+@pytest.mark.parametrize("algo", ["marker", "tile"])
+def test_sh_sphere_512_solid_morton(oracle, gpu, algo):
+    from cuda_voxelizer_amd import meshgen
+    v, f = meshgen.icosphere_sh()
+    got, exp = _run(oracle, gpu, meshgen.to_aos(v, f), 512, True, True, algo)
+    assert _mismatch(got, exp) == 0
