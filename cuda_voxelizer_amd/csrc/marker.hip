@@ -217,6 +217,38 @@ __global__ void __launch_bounds__(256) suffix_rows_kernel(uint32_t* __restrict__
     }
 }
 
+// In-place suffix for G % 128 == 0: a lane holds 4 consecutive words (one
+// 16-byte load), a row is rw/4 <= 64 lanes, a wave holds 256/rw rows.
+__global__ void __launch_bounds__(256) suffix_rows4_kernel(uint32_t* __restrict__ table, uint32_t G,
+                                                           uint64_t nrows) {
+    const uint32_t lpr = G >> 7;  // lanes per row
+    const uint32_t per = 64 / lpr;
+    const int lane = __lane_id();
+    const uint32_t r_in = (uint32_t)lane / lpr, j = (uint32_t)lane % lpr;
+    const bool active_lane = r_in < per;
+    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    uint4* t4 = (uint4*)table;
+    for (uint64_t r0 = wave * per; r0 < nrows; r0 += nwaves * per) {
+        const uint64_t r = r0 + r_in;
+        const bool act = active_lane && r < nrows;
+        const uint64_t idx = r * lpr + j;
+        uint4 v = act ? t4[idx] : make_uint4(0u, 0u, 0u, 0u);
+        const uint32_t pa = __popc(v.x) & 1, pb = __popc(v.y) & 1, pc = __popc(v.z) & 1, pd = __popc(v.w) & 1;
+        const unsigned long long par = __ballot(act && (pa ^ pb ^ pc ^ pd));
+        const uint32_t seg0 = r_in * lpr;
+        const unsigned long long above =
+            (active_lane && j + 1 < lpr) ? ((par >> (seg0 + j + 1)) & ((1ull << (lpr - 1 - j)) - 1ull)) : 0ull;
+        const uint32_t cw = (uint32_t)(__popcll(above) & 1);  // parity of the lanes of larger x
+        const uint32_t cz = cw ^ pd, cy = cz ^ pc, cx = cy ^ pb;
+        v.w = prefix_xor_up(v.w) ^ (0u - cw);
+        v.z = prefix_xor_up(v.z) ^ (0u - cz);
+        v.y = prefix_xor_up(v.y) ^ (0u - cy);
+        v.x = prefix_xor_up(v.x) ^ (0u - cx);
+        if (act) t4[idx] = v;
+    }
+}
+
 // Rows of the linear marker table -> output table, one workgroup per block
 // of 32 rows in y x rz rows in z (all x), staged in LDS.  Linear output: the
 // final words are XORed into (accumulate) or stored to the block's
@@ -424,7 +456,13 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
         solid_marker_kernel<<<(unsigned)blocks, kMBlock, 0, stream>>>(info, tris, markers, ws, offrow,
                                                                        linear ? 1 : 0);
     }
-    if (in_place) {
+    if (in_place && (G & 127u) == 0 && (((uintptr_t)table) & 15u) == 0) {
+        const uint64_t nrows = (uint64_t)G * G;
+        const uint64_t per = 64 / (G >> 7);
+        uint64_t blocks = (nrows / per + 3) / 4;
+        if (blocks > 262144) blocks = 262144;
+        suffix_rows4_kernel<<<(unsigned)blocks, 256, 0, stream>>>(table, G, nrows);
+    } else if (in_place) {
         const uint64_t nrows = (uint64_t)G * G;
         const uint32_t rw = G >> 5;
         const uint64_t rows_per_wave = rw <= 64 ? 64 / rw : 1;
