@@ -3,8 +3,8 @@
 // One workgroup takes a chunk of 256 consecutive triangles (meshes are stored
 // with spatially coherent triangle order, so a chunk covers a compact
 // region).  The chunk is staged through LDS with coalesced loads; each lane
-// rasterizes its own triangle row by row (the YZ test once per row, then the
-// plane, XY and ZX tests per voxel; raster.h row_mask) and ORs the passing
+// rasterizes its own triangle (ZX edges once per z slice, then per row the YZ
+// edges and, at the x that passed ZX, the plane and XY edges) and ORs the passing
 // bits into an LDS hash table keyed by table word, so bits that several
 // triangles of the chunk set in one word are merged before memory is
 // touched.  Triangles with a large voxel box are queued and their rows split
@@ -53,13 +53,12 @@ __device__ __forceinline__ void word_or(ChunkShared& S, uint32_t* __restrict__ t
     atomicOr(table + key, mask);  // hash table crowded: go to memory directly
 }
 
-// Row (y, z) voxels x0..x1 inside the 32-voxel chunk at cx -> hash.
-__device__ __forceinline__ void chunk_row(ChunkShared& S, uint32_t* __restrict__ table, const SurfTri& T,
-                                          const float* xy, const float* zx, const vox_info& info, int layout,
-                                          int x0, int x1, int cx, int y, int z, float py, float pz) {
-    const uint32_t G = info.gridsize[0];
-    const uint32_t mask = row_mask(T.n, T.d1, T.d2, xy, zx, info.unit[0], py, pz, x0, x1, cx);
+// Passing voxels of row (y, z) as a mask (bit k = voxel cx + k, cx a multiple
+// of 32) -> hash, in the table's layout.
+__device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__ table, const vox_info& info,
+                                          int layout, int cx, int y, int z, uint32_t mask) {
     if (!mask) return;
+    const uint32_t G = info.gridsize[0];
     if (layout == VOX_LAYOUT_MORTON) {
         // codes of grids up to 1024 fit 30 bits: 32-bit spreads
         const uint64_t base = G <= 1024u ? (uint64_t)(spread10((uint32_t)cx) | (spread10((uint32_t)y) << 1) |
@@ -87,6 +86,77 @@ __device__ __forceinline__ void chunk_row(ChunkShared& S, uint32_t* __restrict__
             bits |= bit_mask(idx);
         }
         if (bits) word_or(S, table, cur, bits);
+    }
+}
+
+// Row (y, z) voxels x0..x1 inside the 32-voxel chunk at cx -> hash.
+__device__ __forceinline__ void chunk_row(ChunkShared& S, uint32_t* __restrict__ table, const SurfTri& T,
+                                          const float* xy, const float* zx, const vox_info& info, int layout,
+                                          int x0, int x1, int cx, int y, int z, float py, float pz) {
+    emit_mask(S, table, info, layout, cx, y, z, row_mask(T.n, T.d1, T.d2, xy, zx, info.unit[0], py, pz, x0, x1, cx));
+}
+
+// A lane's own box (x extent <= 32): the ZX edge tests depend on (z, x) only,
+// so they are evaluated once per z slice into a mask over the box's x range;
+// a row then tests the XY edges and the plane only at the x that passed ZX.
+// Every test is the same binary32 expression as in vox_device.h, evaluated
+// for a subset of the voxels: the resulting bits are unchanged.
+__device__ __forceinline__ void lane_raster(ChunkShared& S, uint32_t* __restrict__ table, const SurfTri& T,
+                                            const vox_info& info, int layout) {
+    const float ux = info.unit[0], uy = info.unit[1], uz = info.unit[2];
+    const int xa = T.lo[0], xb = T.hi[0];
+    const int c0 = xa & ~31, sh = xa - c0;
+    for (int z = T.lo[2]; z <= T.hi[2]; z++) {
+        const float pz = (float)z * uz;
+        float azx[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) azx[i] = T.zx.a[i] * pz;
+        uint32_t zxm = 0;  // bit i: x = xa + i passes the ZX edges
+        for (int x = xa; x <= xb; x++) {
+            const float px = (float)x * ux;
+            bool ok = true;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                float b = azx[i];
+                b = b + T.zx.b[i] * px;
+                b = b + T.zx.d[i];
+                ok = ok && !(b < 0.0f);
+            }
+            if (ok) zxm |= 1u << (x - xa);
+        }
+        if (!zxm) continue;
+        const float nzpz = T.n[2] * pz;
+        for (int y = T.lo[1]; y <= T.hi[1]; y++) {
+            const float py = (float)y * uy;
+            if (!edges_pass(T.yz, py, pz)) continue;
+            const float nypy = T.n[1] * py;
+            float bxy[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) bxy[i] = T.xy.b[i] * py;
+            uint32_t m = 0, cand = zxm;
+            while (cand) {
+                const int k = __ffs(cand) - 1;
+                cand &= cand - 1;
+                const float px = (float)(xa + k) * ux;
+                float t = T.n[0] * px;
+                t = t + nypy;
+                t = t + nzpz;
+                const float r = (t + T.d1) * (t + T.d2);
+                bool ok = !(r > 0.0f);
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    float a = T.xy.a[i] * px;
+                    a = a + bxy[i];
+                    a = a + T.xy.d[i];
+                    ok = ok && !(a < 0.0f);
+                }
+                if (ok) m |= 1u << k;
+            }
+            if (!m) continue;
+            const uint64_t M = (uint64_t)m << sh;  // bit k = voxel c0 + k
+            emit_mask(S, table, info, layout, c0, y, z, (uint32_t)M);
+            if (M >> 32) emit_mask(S, table, info, layout, c0 + 32, y, z, (uint32_t)(M >> 32));
+        }
     }
 }
 
@@ -150,7 +220,10 @@ __global__ void __launch_bounds__(kChunk) surface_chunk_kernel(ItemDev one, Item
                 queued = true;
             }
         }
-        if (!queued) {
+        if (!queued && nx <= 32) {
+            lane_raster(S, table, T, info, layout);
+        } else if (!queued) {
+            // the queue is full: a wide box stays in its lane, chunk by chunk
             float xy[9], zx[9];
             tri_coeffs(T, xy, zx);
             for (int z = T.lo[2]; z <= T.hi[2]; z++) {
