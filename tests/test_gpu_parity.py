@@ -267,3 +267,25 @@ def test_sh_sphere_512_solid_morton(oracle, gpu, algo):
     v, f = meshgen.icosphere_sh()
     got, exp = _run(oracle, gpu, meshgen.to_aos(v, f), 512, True, True, algo)
     assert _mismatch(got, exp) == 0
+
+
+def test_solid_marker_scratch_reuse(oracle, gpu):
+    """Consecutive marker-algorithm calls on one stream share a scratch marker
+    table that each call must leave all zero: grids and layouts that shrink and
+    grow, with accumulation in between."""
+    import torch
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import meshgen
+    v, f = meshgen.icosphere_sh(level=4, seed=3)
+    tris = meshgen.to_aos(v, f)
+    for grid, morton, acc in [(128, False, False), (64, True, False), (96, False, True), (128, True, True),
+                              (32, False, False), (128, False, False)]:
+        info = V.voxinfo_for_mesh(tris.reshape(-1, 3), grid, tris.shape[0])
+        d = _dev_tris(tris, gpu)
+        base = np.random.default_rng(grid).integers(0, 2**32, V.table_words(grid), dtype=np.uint64).astype(np.uint32)
+        t = torch.from_numpy(base.view(np.int32).copy()).to(gpu)
+        V.voxelize_solid_table(info, d, morton=morton, table=t, accumulate=acc, algo="marker")
+        exp, _, _ = oracle.solid(_oracle_info(oracle, info), tris, morton=morton)
+        if acc:
+            exp = exp ^ base
+        assert _mismatch(_host_table(t), exp) == 0, (grid, morton, acc)
