@@ -258,15 +258,16 @@ __global__ void __launch_bounds__(256) suffix_rows4_kernel(uint32_t* __restrict_
 __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __restrict__ markers,
                                                            uint32_t* __restrict__ table, uint32_t G, uint32_t rz,
                                                            int layout, int accumulate) {
-    extern __shared__ uint32_t tile[];  // [rz * 32 rows][rw]
+    extern __shared__ uint32_t tile[];  // [rz * 32 rows][rw + 1]: the pad word spreads rows over the banks
     const uint32_t rw = G >> 5, nby = G >> 5;
+    const uint32_t ts = rw + 1;  // row stride in LDS
     const uint32_t rb = blockIdx.x;
     const uint32_t nrows = 32u * rz;
     const uint32_t z0 = (rb / nby) * rz, y0 = (rb % nby) * 32u;
     // rows (y0.., z) for each z of the block are 32 * rw contiguous words
     for (uint32_t i = threadIdx.x; i < nrows * rw; i += 256) {
         const uint32_t zl = i / (32u * rw), k = i % (32u * rw);
-        tile[i] = markers[((uint64_t)(z0 + zl) * G + y0) * rw + k];
+        tile[(i / rw) * ts + i % rw] = markers[((uint64_t)(z0 + zl) * G + y0) * rw + k];
     }
     __syncthreads();
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
@@ -278,7 +279,7 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
         for (uint32_t r0 = (uint32_t)wave * per; r0 < nrows; r0 += 4 * per) {
             const uint32_t r = r0 + r_in;
             const bool act = active_lane && r < nrows;
-            const uint32_t w = act ? tile[r * rw + j] : 0u;
+            const uint32_t w = act ? tile[r * ts + j] : 0u;
             const unsigned long long par = __ballot(act && (__popc(w) & 1));
             const uint32_t seg0 = r_in * rw;
             const unsigned long long above =
@@ -289,7 +290,7 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
                     const uint64_t idx = ((uint64_t)(z0 + r / 32u) * G + y0 + (r & 31u)) * rw + j;
                     table[idx] = accumulate ? (table[idx] ^ fin) : fin;
                 } else {
-                    tile[r * rw + j] = fin;
+                    tile[r * ts + j] = fin;
                 }
             }
         }
@@ -300,7 +301,7 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
                 const int64_t lo = hi > 64 ? hi - 64 : 0;
                 const bool act = (int64_t)lane < hi - lo;
                 const uint32_t j = (uint32_t)lo + (uint32_t)lane;
-                const uint32_t w = act ? tile[r * rw + j] : 0u;
+                const uint32_t w = act ? tile[r * ts + j] : 0u;
                 const unsigned long long par = __ballot(act && (__popc(w) & 1));
                 const unsigned long long above = lane < 63 ? (par >> (lane + 1)) : 0ull;
                 const uint32_t fin = prefix_xor_up(w) ^ (((__popcll(above) + carry) & 1) ? 0xFFFFFFFFu : 0u);
@@ -310,7 +311,7 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
                         const uint64_t idx = ((uint64_t)(z0 + r / 32u) * G + y0 + (r & 31u)) * rw + j;
                         table[idx] = accumulate ? (table[idx] ^ fin) : fin;
                     } else {
-                        tile[r * rw + j] = fin;
+                        tile[r * ts + j] = fin;
                     }
                 }
             }
@@ -318,6 +319,59 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
     }
     if (linear) return;
     __syncthreads();
+    if (rz == 8u && (((uintptr_t)table) & 15u) == 0) {
+        // 32 y x 8 z rows: the block's Morton words, with the code bits above
+        // bit 5 fixed by (y0, z0), vary in code bits 5..10 (z1 x2 y2 z2 x3 y3),
+        // 12, 13 (x4 y4) and 3k for x_k, k >= 5.  A thread takes 8 words that
+        // are consecutive in memory (code bits 5..7 = z1 x2 y2) and stores them
+        // as two 16-byte writes; its index supplies z2 x3 y3 x4 y4 x5.. , so
+        // eight neighbouring lanes write 256 contiguous bytes.  Its 8 words
+        // draw on 32 rows (y0 y1 y2 z0 z1 free) and one byte (8 x) of each.
+        const uint64_t base_w = morton3(0u, y0, z0) >> 5;
+        for (uint32_t t = threadIdx.x; t < G; t += 256) {
+            const uint32_t z2 = t & 1u, x3 = (t >> 1) & 1u, y3 = (t >> 2) & 1u, x4 = (t >> 3) & 1u,
+                           y4 = (t >> 4) & 1u, xh = t >> 5;
+            const uint32_t xbyte = x3 | (x4 << 1) | (xh << 2);  // x >> 3
+            const uint32_t j = xbyte >> 2, bsh = 24u - 8u * (xbyte & 3u);
+            uint32_t bytes[32];  // row (y0 y1 y2 z0 z1) -> its byte of x
+#pragma unroll
+            for (int q = 0; q < 32; q++) {
+                const uint32_t y_l = ((uint32_t)q & 7u) | (y3 << 3) | (y4 << 4);
+                const uint32_t z_l = (((uint32_t)q >> 3) & 3u) | (z2 << 2);
+                bytes[q] = (tile[(z_l * 32u + y_l) * ts + j] >> bsh) & 0xFFu;
+            }
+            uint32_t out[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {  // k = z1 | x2 << 1 | y2 << 2
+                const uint32_t z1 = (uint32_t)k & 1u, x2 = ((uint32_t)k >> 1) & 1u, y2 = ((uint32_t)k >> 2) & 1u;
+                uint32_t word = 0;
+#pragma unroll
+                for (int r = 0; r < 8; r++) {  // r = y0 | y1 << 1 | z0 << 2
+                    const uint32_t y0b = (uint32_t)r & 1u, y1b = ((uint32_t)r >> 1) & 1u, z0b = ((uint32_t)r >> 2) & 1u;
+                    const uint32_t q = (y0b | (y1b << 1) | (y2 << 2)) | ((z0b | (z1 << 1)) << 3);
+                    // byte bit 7 = x offset 0; x2 selects the nibble; nibble bits 3..0 = x0 + 2 x1 = 0..3
+                    const uint32_t nib = x2 ? (bytes[q] & 0xFu) : (bytes[q] >> 4);
+                    const uint32_t c0 = (y0b << 1) | (z0b << 2) | (y1b << 4);
+                    word |= ((nib >> 2) << (30u - c0)) | ((nib & 3u) << (22u - c0));
+                }
+                out[k] = word;
+            }
+            // word offset inside the block's code range: bits 3 4 5 7 8 from t, then x5.. at 10, 13, ...
+            uint64_t wl = ((uint64_t)z2 << 3) | ((uint64_t)x3 << 4) | ((uint64_t)y3 << 5) | ((uint64_t)x4 << 7) |
+                          ((uint64_t)y4 << 8);
+            for (uint32_t b = 0; (xh >> b) != 0u; b++) wl |= (uint64_t)((xh >> b) & 1u) << (10u + 3u * b);
+            uint4* dst = (uint4*)(table + base_w + wl);
+            uint4 lo = make_uint4(out[0], out[1], out[2], out[3]), hi = make_uint4(out[4], out[5], out[6], out[7]);
+            if (accumulate) {
+                const uint4 a = dst[0], b = dst[1];
+                lo.x ^= a.x; lo.y ^= a.y; lo.z ^= a.z; lo.w ^= a.w;
+                hi.x ^= b.x; hi.y ^= b.y; hi.z ^= b.z; hi.w ^= b.w;
+            }
+            dst[0] = lo;
+            dst[1] = hi;
+        }
+        return;
+    }
     const uint32_t groups = 8u * (rz / 2u);  // brick rows (4 y x 2 z) in the block
     for (uint32_t t = threadIdx.x; t < groups * rw; t += 256) {
         const uint32_t j = t % rw, br = t / rw;
@@ -326,7 +380,7 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
 #pragma unroll
         for (int q = 0; q < 8; q++) {
             const uint32_t y0b = (uint32_t)q & 1u, z0b = ((uint32_t)q >> 1) & 1u, y1b = ((uint32_t)q >> 2) & 1u;
-            row_w[q] = tile[((zb * 2u + z0b) * 32u + yb * 4u + y0b + 2u * y1b) * rw + j];
+            row_w[q] = tile[((zb * 2u + z0b) * 32u + yb * 4u + y0b + 2u * y1b) * ts + j];
         }
 #pragma unroll
         for (int xg = 0; xg < 8; xg++) {
@@ -473,7 +527,7 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
     } else {
         const uint32_t rz = linear ? 1u : (G <= 1024u ? 8u : 2u);
         const uint32_t nblocks = (G / 32u) * (G / rz);
-        marker_build_kernel<<<nblocks, 256, 4ull * 32u * rz * (G / 32u), stream>>>(markers, table, G, rz, layout,
+        marker_build_kernel<<<nblocks, 256, 4ull * 32u * rz * (G / 32u + 1u), stream>>>(markers, table, G, rz, layout,
                                                                                  accumulate ? 1 : 0);
     }
     if (linear) {
