@@ -136,7 +136,9 @@ vox_status run(const vox_info* info, const float* d_tris, uint32_t* d_table, int
     else if (flags & VOX_FLAG_ALGO_TILE) algo = have_plan ? TILE : DIRECT;
     else if (flags & VOX_FLAG_ALGO_CHUNK) algo = chunk_supported(G) ? CHUNK : DIRECT;
     else if (flags & VOX_FLAG_ALGO_MARKER) algo = have_marker ? MARKER : (have_plan ? TILE : DIRECT);
+    else if (solid && have_plan && prefer_tile_surface(info->n_triangles, G)) algo = TILE;
     else if (solid) algo = have_marker ? MARKER : (have_plan ? TILE : DIRECT);
+    else if (have_plan && prefer_tile_surface(info->n_triangles, G)) algo = TILE;
     else algo = chunk_supported(G) ? CHUNK : (have_plan ? TILE : DIRECT);
     hipError_t e;
     if (algo == MARKER) {
