@@ -24,7 +24,7 @@ constexpr int kChunk = 256;  // triangles per workgroup (one per lane)
 constexpr int kHashBits = 11;
 constexpr int kHashCap = 1 << kHashBits;
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
-constexpr int kQueue = 32;      // big triangles split over the workgroup
+constexpr int kQueue = kChunk;  // big triangles split over the workgroup (room for all of them)
 constexpr int kBigVox = 256;    // voxel boxes above this (or wider than 32 in x) are queued
 
 struct ChunkShared {
@@ -211,33 +211,13 @@ __global__ void __launch_bounds__(kChunk) surface_chunk_kernel(ItemDev one, Item
         SurfTri T;
         surface_setup(v, info, T);
         const int nx = T.hi[0] - T.lo[0] + 1;
-        const int vol = nx * (T.hi[1] - T.lo[1] + 1) * (T.hi[2] - T.lo[2] + 1);
+        const int64_t vol = (int64_t)nx * (T.hi[1] - T.lo[1] + 1) * (T.hi[2] - T.lo[2] + 1);
         bool queued = false;
         if (vol > kBigVox || nx > 32) {
-            const unsigned q = atomicAdd(&S.qn, 1u);
-            if (q < (unsigned)kQueue) {
-                S.queue[q] = threadIdx.x;
-                queued = true;
-            }
+            S.queue[atomicAdd(&S.qn, 1u)] = threadIdx.x;  // at most kChunk entries
+            queued = true;
         }
-        if (!queued && nx <= 32) {
-            lane_raster(S, table, T, info, layout);
-        } else if (!queued) {
-            // the queue is full: a wide box stays in its lane, chunk by chunk
-            float xy[9], zx[9];
-            tri_coeffs(T, xy, zx);
-            for (int z = T.lo[2]; z <= T.hi[2]; z++) {
-                const float pz = (float)z * info.unit[2];
-                for (int y = T.lo[1]; y <= T.hi[1]; y++) {
-                    const float py = (float)y * info.unit[1];
-                    if (!edges_pass(T.yz, py, pz)) continue;
-                    for (int cx = T.lo[0] & ~31; cx <= T.hi[0]; cx += 32) {
-                        const int x0 = cx > T.lo[0] ? cx : T.lo[0], x1 = cx + 31 < T.hi[0] ? cx + 31 : T.hi[0];
-                        chunk_row(S, table, T, xy, zx, info, layout, x0, x1, cx, y, z, py, pz);
-                    }
-                }
-            }
-        }
+        if (!queued) lane_raster(S, table, T, info, layout);  // nx <= 32 here
     }
     __syncthreads();
     // queued triangles: rows x 32-voxel chunks split over the workgroup
@@ -251,10 +231,11 @@ __global__ void __launch_bounds__(kChunk) surface_chunk_kernel(ItemDev one, Item
         tri_coeffs(T, xy, zx);
         const int ny = T.hi[1] - T.lo[1] + 1, nz = T.hi[2] - T.lo[2] + 1;
         const int c0 = T.lo[0] >> 5, nch = (T.hi[0] >> 5) - c0 + 1;
-        const int items = ny * nz * nch;
-        for (int i = threadIdx.x; i < items; i += kChunk) {
-            const int ch = i % nch, r = i / nch;
-            const int y = T.lo[1] + r % ny, z = T.lo[2] + r / ny;
+        const int64_t items = (int64_t)ny * nz * nch;
+        for (int64_t i = threadIdx.x; i < items; i += kChunk) {
+            const int ch = (int)(i % nch);
+            const int64_t r = i / nch;
+            const int y = T.lo[1] + (int)(r % ny), z = T.lo[2] + (int)(r / ny);
             const float py = (float)y * info.unit[1], pz = (float)z * info.unit[2];
             if (!edges_pass(T.yz, py, pz)) continue;
             const int cx = (c0 + ch) << 5;

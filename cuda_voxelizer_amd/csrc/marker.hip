@@ -38,7 +38,7 @@
 namespace voxamd {
 
 constexpr int kMBlock = 256;
-constexpr int kMQueue = 32;        // triangles with many columns, split over the workgroup
+constexpr int kMQueue = kMBlock;   // triangles with many columns, split over the workgroup
 constexpr int kMBigCols = 64;      // columns above which a triangle is queued
 constexpr uint32_t kSpillCap = 1u << 16;
 
