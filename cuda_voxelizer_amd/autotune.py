@@ -1,13 +1,25 @@
-"""Per-class algorithm autotuner.
+"""Per-class algorithm autotuner for batched launches.
 
-A voxelization job's best algorithm (chunk / tile / direct, all bit-identical)
-depends on its grid, layout, mode and size: chunk wins on coherent surface
-meshes with little word sharing across chunks, tile on large grids and solid
-fills, direct on tiny jobs where one launch of one-triangle-per-lane work is
-all there is.  ``Tuner.choose`` times every applicable algorithm once per class
-(grid, layout, surface/solid, log2 size bucket) on the job it is asked about,
-keeps the fastest, and remembers it in a JSON cache keyed by the device name,
-so later jobs and later processes reuse the decision.
+A voxelization job's best algorithm (chunk / marker / tile / direct, all
+bit-identical) depends on its grid, layout, mode and size: chunk and marker
+win on fine meshes, tile on coarse ones (few large triangles), direct on fine
+solid meshes at small grids.  ``Tuner.choose`` times
+every applicable algorithm once per class (grid, layout, surface/solid, log2
+size bucket) on the job it is asked about, keeps the fastest, and remembers it
+in a JSON cache keyed by the device name, so later jobs and later processes
+reuse the decision.
+
+What is timed is the job's cost inside a batch, not alone: a job of a few
+hundred triangles runs a handful of workgroups, so alone it measures launch
+latency and favours whichever algorithm spreads one mesh over the most
+workgroups, while in a batch the other meshes fill the device and only the
+work counts.  Each candidate runs as one graph launch of K copies of the job
+(each with its own table; K sized so the copies hold about ``fill_triangles``
+triangles, 4..64, and their tables stay under ``max_table_bytes``), giving
+the time per copy, and as a launch of one copy, giving the job's critical
+path: a batch lasts at least as long as its slowest job, so a candidate whose
+single-copy time exceeds ``latency_slack`` times the best one (plus 20 us of
+launch noise) is ruled out, and the fastest per copy among the rest wins.
 
 Cache file: $VOX_AUTOTUNE_CACHE, else ~/.cache/cuda_voxelizer_amd/autotune.json.
 """
@@ -19,7 +31,7 @@ import os
 import time
 
 SURFACE_ALGOS = ("chunk", "tile", "direct")
-SOLID_ALGOS = ("tile", "direct")
+SOLID_ALGOS = ("marker", "tile", "direct")
 
 
 def default_cache_path() -> str:
@@ -35,9 +47,13 @@ def class_key(grid: int, morton: bool, solid: bool, n_triangles: int) -> str:
 
 
 class Tuner:
-    def __init__(self, path: str | None = None, device_name: str | None = None, reps: int = 3, timer=None):
+    def __init__(self, path: str | None = None, device_name: str | None = None, reps: int = 3, timer=None,
+                 fill_triangles: int = 2_000_000, max_table_bytes: int = 512 << 20, latency_slack: float = 2.0):
         self.path = path or default_cache_path()
         self.reps = reps
+        self.fill_triangles = fill_triangles
+        self.max_table_bytes = max_table_bytes
+        self.latency_slack = latency_slack
         self._timer = timer  # callable(job, algo) -> seconds; default times on the GPU
         self.device_name = device_name or self._device_name()
         self.table: dict = {}
@@ -72,21 +88,46 @@ class Tuner:
             json.dump(doc, fh, indent=1, sort_keys=True)
         os.replace(tmp, self.path)
 
-    def _time_gpu(self, job, algo: str) -> float:
+    def copies(self, job) -> int:
+        """K: copies of `job` in one timed batch launch."""
+        n = max(1, int(job.info.n_triangles))
+        k = min(64, max(4, -(-self.fill_triangles // n)))
+        table_bytes = max(1, int(job.table.numel()) * job.table.element_size())
+        return max(1, min(k, self.max_table_bytes // table_bytes))
+
+    def _time_batch(self, job, algo: str, k: int) -> float:
         import torch
 
-        from . import voxelize_solid_table, voxelize_surface
-        fn = voxelize_solid_table if job.solid else voxelize_surface
-        table = torch.empty_like(job.table)
-        fn(job.info, job.tris, morton=job.morton, table=table, algo=algo)  # warm (workspace, code load)
-        torch.cuda.synchronize()
-        best = float("inf")
-        for _ in range(self.reps):
-            t0 = time.perf_counter()
-            fn(job.info, job.tris, morton=job.morton, table=table, algo=algo)
+        from . import Batch, BatchJob
+        tables = [torch.empty_like(job.table) for _ in range(k)]
+        copies = [BatchJob(job.info, job.tris, t, morton=job.morton, solid=job.solid, algo=algo) for t in tables]
+        batch = Batch(copies, use_graph=True)
+        try:
+            batch.launch()  # warm (code load, workspace)
             torch.cuda.synchronize()
-            best = min(best, time.perf_counter() - t0)
+            best = float("inf")
+            for _ in range(self.reps):
+                t0 = time.perf_counter()
+                batch.launch()
+                torch.cuda.synchronize()
+                best = min(best, time.perf_counter() - t0)
+        finally:
+            batch.close()
         return best
+
+    def _time_gpu(self, job, algo: str) -> tuple:
+        """(single-copy seconds, seconds per copy in a K-copy launch)."""
+        k = self.copies(job)
+        return self._time_batch(job, algo, 1), self._time_batch(job, algo, k) / k
+
+    def pick(self, times: dict) -> str:
+        """Fastest per copy among the candidates whose single-copy time is
+        within latency_slack of the best.  A plain number is both times."""
+        single = {a: (t[0] if isinstance(t, (tuple, list)) else t) for a, t in times.items()}
+        per = {a: (t[1] if isinstance(t, (tuple, list)) else t) for a, t in times.items()}
+        bound = self.latency_slack * min(single.values()) + 20e-6
+        ok = [a for a in times if single[a] <= bound]
+        return min(ok, key=per.get)
 
     def choose(self, job) -> str:
         """Best algorithm for `job` (a BatchJob); times the candidates on a miss."""
@@ -96,8 +137,9 @@ class Tuner:
             return hit["algo"]
         timer = self._timer or self._time_gpu
         times = {a: timer(job, a) for a in (SOLID_ALGOS if job.solid else SURFACE_ALGOS)}
-        algo = min(times, key=times.get)
-        self.table[key] = {"algo": algo, "seconds": times}
+        algo = self.pick(times)
+        self.table[key] = {"algo": algo, "seconds": {a: list(t) if isinstance(t, tuple) else t
+                                                     for a, t in times.items()}}
         return algo
 
     def tune_jobs(self, jobs) -> list:

@@ -1,7 +1,7 @@
 // batch.hip -- many voxelizations in one graph launch (vox_batch_* C-ABI).
 //
 // vox_batch_create sorts the jobs into algorithm classes (fill, chunk,
-// direct surface/solid, tile surface/solid), uploads one device array of job
+// marker, direct surface/solid, tile surface/solid), uploads one device array of job
 // descriptors per class together with the per-job prefix of each kernel's
 // work units, allocates one workspace, and captures the whole pipeline --
 // every kernel processing all jobs of its class at once -- into a HIP graph.
@@ -26,6 +26,7 @@ struct JobClass {
     ItemMap m_a{}, m_b{}, m_c{};
     uint32_t total_a = 0, total_b = 0, total_c = 0;
     uint32_t max_tile_words = 0;
+    bool loop = false;  // chunk class: some job takes several chunks per workgroup
     size_t off_items = 0, off_a = 0, off_b = 0, off_c = 0;
 };
 
@@ -39,8 +40,9 @@ struct vox_batch {
     hipGraphExec_t exec = nullptr;
     void* dev_mem = nullptr;
     void* ws = nullptr;
+    void* mws = nullptr;  // marker class: spill records and counters
     voxamd::WorkspaceLayout L{};
-    voxamd::JobClass fill, chunk, dsurf, dsolid, tsurf, tsolid;
+    voxamd::JobClass fill, chunk, marker, dsurf, dsolid, tsurf, tsolid;
     int count = 0;
     uint64_t total_triangles = 0;
 };
@@ -57,7 +59,13 @@ static hipError_t launch_all(vox_batch* b, hipStream_t s) {
     const ItemDev dummy{};
     hipError_t e = hipSuccess;
     if (b->fill.total_a && (e = launch_fill_jobs(dummy, b->fill.m_a, b->fill.total_a, s)) != hipSuccess) return e;
-    if (b->chunk.total_a && (e = launch_chunk_jobs(dummy, b->chunk.m_a, b->chunk.total_a, s)) != hipSuccess) return e;
+    if (b->chunk.total_a &&
+        (e = launch_chunk_jobs(dummy, b->chunk.m_a, b->chunk.total_a, b->chunk.loop, s)) != hipSuccess)
+        return e;
+    if (!b->marker.items.empty() &&
+        (e = launch_marker_jobs(b->marker.m_a, b->marker.m_b, b->marker.m_c, b->marker.total_a, b->marker.total_b,
+                                b->mws, s)) != hipSuccess)
+        return e;
     if (b->dsurf.total_a && (e = launch_direct_jobs(dummy, b->dsurf.m_a, b->dsurf.total_a, false, s)) != hipSuccess)
         return e;
     if (b->dsolid.total_a && (e = launch_direct_jobs(dummy, b->dsolid.m_a, b->dsolid.total_a, true, s)) != hipSuccess)
@@ -80,6 +88,7 @@ static void free_batch(vox_batch* b) {
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
     if (b->dev_mem) (void)hipFree(b->dev_mem);
     if (b->ws) (void)hipFree(b->ws);
+    if (b->mws) (void)hipFree(b->mws);
     delete b;
 }
 
@@ -92,7 +101,7 @@ struct BatchPlan {
 };
 
 static void reset_classes(vox_batch* b) {
-    JobClass* classes[] = {&b->fill, &b->chunk, &b->dsurf, &b->dsolid, &b->tsurf, &b->tsolid};
+    JobClass* classes[] = {&b->fill, &b->chunk, &b->marker, &b->dsurf, &b->dsolid, &b->tsurf, &b->tsolid};
     for (JobClass* c : classes) *c = JobClass{};
     b->count = 0;
     b->total_triangles = 0;
@@ -107,18 +116,23 @@ static vox_status plan_batch(const vox_batch_item* items, int count, vox_batch* 
         const vox_batch_item& it = items[i];
         vox_status s = validate(&it.info, it.d_tris, it.d_table, it.layout);
         if (s != VOX_OK) return fail(s, "item " + std::to_string(i) + ": " + vox_last_error());
-        const unsigned algo = it.flags & (VOX_FLAG_ALGO_DIRECT | VOX_FLAG_ALGO_TILE | VOX_FLAG_ALGO_CHUNK);
+        const unsigned algo =
+            it.flags & (VOX_FLAG_ALGO_DIRECT | VOX_FLAG_ALGO_TILE | VOX_FLAG_ALGO_CHUNK | VOX_FLAG_ALGO_MARKER);
         const bool solid = it.solid != 0;
-        if ((algo & (algo - 1)) || (solid && (algo & VOX_FLAG_ALGO_CHUNK)))
+        if ((algo & (algo - 1)) || (solid && (algo & VOX_FLAG_ALGO_CHUNK)) || (!solid && (algo & VOX_FLAG_ALGO_MARKER)))
             return fail(VOX_ERR_INVALID, "item " + std::to_string(i) + ": bad algorithm flags");
         const uint32_t G = it.info.gridsize[0];
         const bool accumulate = (it.flags & VOX_FLAG_ACCUMULATE) != 0;
         TilePlan P;
         const bool have_plan = make_plan((int)G, it.layout, solid, P);
-        enum { DIRECT, TILE, CHUNK } a;
+        // the marker class tags spill records with the job index (below 2^22)
+        const bool have_marker = solid && marker_batch_supported(G, it.layout, accumulate) && i < (1 << 22);
+        enum { DIRECT, TILE, CHUNK, MARKER } a;
         if (algo & VOX_FLAG_ALGO_DIRECT) a = DIRECT;
         else if (algo & VOX_FLAG_ALGO_TILE) a = have_plan ? TILE : DIRECT;
         else if (algo & VOX_FLAG_ALGO_CHUNK) a = chunk_supported(G) ? CHUNK : DIRECT;
+        else if (algo & VOX_FLAG_ALGO_MARKER) a = have_marker ? MARKER : (have_plan ? TILE : DIRECT);
+        else if (solid && have_marker && !(have_plan && prefer_tile_surface(it.info.n_triangles, G))) a = MARKER;
         else if (solid) a = have_plan ? TILE : DIRECT;
         else if (have_plan && prefer_tile_surface(it.info.n_triangles, G)) a = TILE;
         else a = chunk_supported(G) ? CHUNK : (have_plan ? TILE : DIRECT);
@@ -136,9 +150,19 @@ static vox_status plan_batch(const vox_batch_item* items, int count, vox_batch* 
             b->fill.items.push_back(d);
             add_prefix(b->fill.base_a, b->fill.total_a, fill_units(G));
         }
-        if (a == CHUNK) {
+        if (a == MARKER) {
+            if ((uint64_t)b->marker.total_c + n > 0xFFFFFFFFull)
+                return fail(VOX_ERR_UNSUPPORTED, "more than 2^32-1 triangles in one marker batch class");
+            b->marker.items.push_back(d);
+            add_prefix(b->marker.base_a, b->marker.total_a, marker_block_units(n));
+            add_prefix(b->marker.base_b, b->marker.total_b, marker_row_units(G));
+            add_prefix(b->marker.base_c, b->marker.total_c, n);  // triangles (spill overflow pass)
+        } else if (a == CHUNK) {
+            d.span = chunk_span(G, n);
+            d.parts = chunk_parts(G, n);
+            if (d.span > 1u && d.parts <= 1u) b->chunk.loop = true;
             b->chunk.items.push_back(d);
-            add_prefix(b->chunk.base_a, b->chunk.total_a, chunk_units(n));
+            add_prefix(b->chunk.base_a, b->chunk.total_a, chunk_units(n, d.span, d.parts));
         } else if (a == DIRECT) {
             JobClass& c = solid ? b->dsolid : b->dsurf;
             c.items.push_back(d);
@@ -160,7 +184,7 @@ static vox_status plan_batch(const vox_batch_item* items, int count, vox_batch* 
         }
     }
     // one device allocation: per class the job array and its prefix arrays
-    JobClass* classes[] = {&b->fill, &b->chunk, &b->dsurf, &b->dsolid, &b->tsurf, &b->tsolid};
+    JobClass* classes[] = {&b->fill, &b->chunk, &b->marker, &b->dsurf, &b->dsolid, &b->tsurf, &b->tsolid};
     size_t total = 0;
     auto reserve = [&](size_t bytes) {
         const size_t off = total;
@@ -193,7 +217,7 @@ static vox_status plan_batch(const vox_batch_item* items, int count, vox_batch* 
 // Point every class's job maps at a device copy of the packed image.
 static void bind_maps(vox_batch* b, void* dev) {
     char* dm = (char*)dev;
-    JobClass* classes[] = {&b->fill, &b->chunk, &b->dsurf, &b->dsolid, &b->tsurf, &b->tsolid};
+    JobClass* classes[] = {&b->fill, &b->chunk, &b->marker, &b->dsurf, &b->dsolid, &b->tsurf, &b->tsolid};
     for (JobClass* c : classes) {
         const ItemDev* it = (const ItemDev*)(dm + c->off_items);
         const int ni = (int)c->items.size();
@@ -237,6 +261,13 @@ vox_status vox_batch_create(const vox_batch_item* items, int count, unsigned fla
         return hip_fail(e, "batch descriptor upload");
     }
     bind_maps(b, b->dev_mem);
+    if (!b->marker.items.empty()) {
+        e = hipMalloc(&b->mws, marker_ws_bytes());  // counters are reset by a kernel at every launch
+        if (e != hipSuccess) {
+            free_batch(b);
+            return fail(VOX_ERR_OOM, std::string("batch marker workspace: ") + hipGetErrorString(e));
+        }
+    }
     if (plan.ws_needed) {
         b->L = plan.L;
         e = hipMalloc(&b->ws, b->L.total);
@@ -360,6 +391,13 @@ vox_status vox_runner_launch(vox_runner* r, const vox_batch_item* items, int cou
             return fail(VOX_ERR_OOM, std::string("runner workspace: ") + hipGetErrorString(e));
         }
     }
+    if (!r->b.marker.items.empty() && !r->b.mws) {  // fixed size, allocated once
+        e = hipMalloc(&r->b.mws, marker_ws_bytes());
+        if (e != hipSuccess) {
+            r->b.mws = nullptr;
+            return fail(VOX_ERR_OOM, std::string("runner marker workspace: ") + hipGetErrorString(e));
+        }
+    }
     if (grow_desc) {
         if (r->dev_desc) (void)hipFree(r->dev_desc);
         r->dev_cap = plan.host.size() * 2;
@@ -398,14 +436,16 @@ vox_status vox_runner_launch(vox_runner* r, const vox_batch_item* items, int cou
 
 vox_status vox_runner_destroy(vox_runner* r) {
     if (!r) return VOX_OK;
-    if (r->stream || r->b.ws || r->dev_desc) (void)hipStreamSynchronize(r->stream);
+    if (r->stream || r->b.ws || r->b.mws || r->dev_desc) (void)hipStreamSynchronize(r->stream);
     for (int i = 0; i < 2; i++) {
         if (r->ev[i]) (void)hipEventDestroy(r->ev[i]);
         if (r->pinned[i]) (void)hipHostFree(r->pinned[i]);
     }
     if (r->dev_desc) (void)hipFree(r->dev_desc);
     if (r->b.ws) (void)hipFree(r->b.ws);
+    if (r->b.mws) (void)hipFree(r->b.mws);
     r->b.ws = nullptr;
+    r->b.mws = nullptr;
     delete r;
     return VOX_OK;
 }

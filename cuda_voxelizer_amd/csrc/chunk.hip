@@ -7,8 +7,11 @@
 // edges and, at the x that passed ZX, the plane and XY edges) and ORs the passing
 // bits into an LDS hash table keyed by table word, so bits that several
 // triangles of the chunk set in one word are merged before memory is
-// touched.  Triangles with a large voxel box are queued and their rows split
-// over the whole workgroup afterwards.  The table is then updated with one
+// touched.  Triangles with a large voxel box are queued; afterwards their
+// owners publish test records to LDS (up to kRecs per round) and the
+// workgroup deals the rounds' (row, 32-voxel x chunk) items evenly over its
+// lanes, testing only the x range that the plane band leaves open.  The
+// table is then updated with one
 // non-returning global atomicOr per distinct word; words that do not fit in
 // the hash go straight to global atomics.  The table is zero-filled
 // beforehand (unless accumulating).
@@ -24,21 +27,32 @@ constexpr int kChunk = 256;  // triangles per workgroup (one per lane)
 constexpr int kHashBits = 11;
 constexpr int kHashCap = 1 << kHashBits;
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
-constexpr int kQueue = kChunk;  // big triangles split over the workgroup (room for all of them)
-constexpr int kBigVox = 256;    // voxel boxes above this (or wider than 32 in x) are queued
+// Slots tried before a word goes straight to memory: a chunk of scattered
+// triangles has more distinct words than slots, and probing a full table costs
+// more than the atomic it would save.
+constexpr int kProbes = 4;
+constexpr int kBigVox = 64;     // voxel boxes above this (or wider than 32 in x) are queued
+constexpr int kRecs = 48;       // queued triangles whose records are in LDS at once
 
 struct ChunkShared {
     uint32_t keys[kHashCap];
     uint32_t vals[kHashCap];
-    float tri[kChunk * 9];
-    unsigned queue[kQueue];
-    unsigned qn;
+    union {
+        float tri[kChunk * 9];  // staged vertices (until every lane has its setup)
+        SurfRec recs[kRecs];    // then: records of the queued triangles of one round
+    };
+    unsigned long long incl[kRecs];  // inclusive prefix of the round's item counts
+    unsigned wq[kChunk / 64];        // queued triangles per wave
 };
+static_assert(sizeof(SurfRec) * kRecs <= sizeof(float) * kChunk * 9, "records must fit the staging area");
 
-__device__ __forceinline__ void word_or(ChunkShared& S, uint32_t* __restrict__ table, uint32_t key, uint32_t mask) {
-    uint32_t h = (key * 2654435761u) >> (32 - kHashBits);
+// dense: the whole table has at most kHashCap words; slot w is preset to key
+// w, so every word is found at its first probe.
+__device__ __forceinline__ void word_or(ChunkShared& S, uint32_t* __restrict__ table, bool dense, uint32_t key,
+                                        uint32_t mask) {
+    uint32_t h = dense ? key : (key * 2654435761u) >> (32 - kHashBits);
 #pragma unroll 1
-    for (int probe = 0; probe < 16; probe++) {
+    for (int probe = 0; probe < kProbes; probe++) {
         uint32_t k = S.keys[h];
         if (k == kEmpty) {
             k = atomicCAS(&S.keys[h], kEmpty, key);
@@ -55,8 +69,8 @@ __device__ __forceinline__ void word_or(ChunkShared& S, uint32_t* __restrict__ t
 
 // Passing voxels of row (y, z) as a mask (bit k = voxel cx + k, cx a multiple
 // of 32) -> hash, in the table's layout.
-__device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__ table, const vox_info& info,
-                                          int layout, int cx, int y, int z, uint32_t mask) {
+__device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__ table, bool dense,
+                                          const vox_info& info, int layout, int cx, int y, int z, uint32_t mask) {
     if (!mask) return;
     const uint32_t G = info.gridsize[0];
     if (layout == VOX_LAYOUT_MORTON) {
@@ -65,10 +79,10 @@ __device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__
                                                       (spread10((uint32_t)z) << 2))
                                          : morton3((uint32_t)cx, (uint32_t)y, (uint32_t)z);
         const uint32_t yz_low = ((uint32_t)(y & 1) << 1) | ((uint32_t)(z & 1) << 2) | ((uint32_t)((y >> 1) & 1) << 4);
-        morton_chunk_words(base, yz_low, mask, [&](uint64_t w, uint32_t bits) { word_or(S, table, (uint32_t)w, bits); });
+        morton_chunk_words(base, yz_low, mask, [&](uint64_t w, uint32_t bits) { word_or(S, table, dense, (uint32_t)w, bits); });
     } else if ((G & 31u) == 0) {
         const uint64_t w = ((uint64_t)cx + (uint64_t)y * G + (uint64_t)z * G * G) >> 5;
-        word_or(S, table, (uint32_t)w, __brev(mask));
+        word_or(S, table, dense, (uint32_t)w, __brev(mask));
     } else {
         // linear rows not aligned to words: per voxel
         const uint64_t row = (uint64_t)y * G + (uint64_t)z * G * G;
@@ -79,21 +93,14 @@ __device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__
             const uint64_t idx = row + (uint64_t)(cx + k);
             const uint32_t w = (uint32_t)(idx >> 5);
             if (w != cur) {
-                if (bits) word_or(S, table, cur, bits);
+                if (bits) word_or(S, table, dense, cur, bits);
                 cur = w;
                 bits = 0;
             }
             bits |= bit_mask(idx);
         }
-        if (bits) word_or(S, table, cur, bits);
+        if (bits) word_or(S, table, dense, cur, bits);
     }
-}
-
-// Row (y, z) voxels x0..x1 inside the 32-voxel chunk at cx -> hash.
-__device__ __forceinline__ void chunk_row(ChunkShared& S, uint32_t* __restrict__ table, const SurfTri& T,
-                                          const float* xy, const float* zx, const vox_info& info, int layout,
-                                          int x0, int x1, int cx, int y, int z, float py, float pz) {
-    emit_mask(S, table, info, layout, cx, y, z, row_mask(T.n, T.d1, T.d2, xy, zx, info.unit[0], py, pz, x0, x1, cx));
 }
 
 // A lane's own box (x extent <= 32): the ZX edge tests depend on (z, x) only,
@@ -101,8 +108,8 @@ __device__ __forceinline__ void chunk_row(ChunkShared& S, uint32_t* __restrict__
 // a row then tests the XY edges and the plane only at the x that passed ZX.
 // Every test is the same binary32 expression as in vox_device.h, evaluated
 // for a subset of the voxels: the resulting bits are unchanged.
-__device__ __forceinline__ void lane_raster(ChunkShared& S, uint32_t* __restrict__ table, const SurfTri& T,
-                                            const vox_info& info, int layout) {
+__device__ __forceinline__ void lane_raster(ChunkShared& S, uint32_t* __restrict__ table, bool dense,
+                                            const SurfTri& T, const vox_info& info, int layout) {
     const float ux = info.unit[0], uy = info.unit[1], uz = info.unit[2];
     const int xa = T.lo[0], xb = T.hi[0];
     const int c0 = xa & ~31, sh = xa - c0;
@@ -154,17 +161,9 @@ __device__ __forceinline__ void lane_raster(ChunkShared& S, uint32_t* __restrict
             }
             if (!m) continue;
             const uint64_t M = (uint64_t)m << sh;  // bit k = voxel c0 + k
-            emit_mask(S, table, info, layout, c0, y, z, (uint32_t)M);
-            if (M >> 32) emit_mask(S, table, info, layout, c0 + 32, y, z, (uint32_t)(M >> 32));
+            emit_mask(S, table, dense, info, layout, c0, y, z, (uint32_t)M);
+            if (M >> 32) emit_mask(S, table, dense, info, layout, c0 + 32, y, z, (uint32_t)(M >> 32));
         }
-    }
-}
-
-__device__ __forceinline__ void tri_coeffs(const SurfTri& T, float xy[9], float zx[9]) {
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        xy[i] = T.xy.a[i]; xy[3 + i] = T.xy.b[i]; xy[6 + i] = T.xy.d[i];
-        zx[i] = T.zx.a[i]; zx[3 + i] = T.zx.b[i]; zx[6 + i] = T.zx.d[i];
     }
 }
 
@@ -177,10 +176,19 @@ __device__ __forceinline__ void staged_vertices(const ChunkShared& S, int lt, co
     }
 }
 
-// Work unit: kChunk consecutive triangles of one job.  MULTI: the job comes
-// from the launch's job map; otherwise it is `one` (kernel arguments).
-template <bool MULTI>
-__global__ void __launch_bounds__(kChunk) surface_chunk_kernel(ItemDev one, ItemMap M) {
+// Work unit: `span` consecutive chunks of kChunk triangles of one job, merged
+// in one LDS table and flushed once -- or, with `parts` > 1, one chunk whose
+// queued items are shared by `parts` workgroups (each does the setup, takes
+// every parts-th slice of the items, and part 0 also the lane rasters; queue
+// slots follow lane order, so every part sees the same items).  MULTI: the job
+// comes from the launch's job map; otherwise it is `one` (kernel arguments).
+// LOOP: some job has span > 1 (otherwise the chunk loop is compiled out, which
+// keeps the register count of the one-chunk form down).
+// Five workgroups per CU: the LDS allows five, and the bound keeps the
+// register count at the matching five waves per SIMD (four for the chunk loop,
+// which would otherwise spill).
+template <bool MULTI, bool LOOP>
+__global__ void __launch_bounds__(kChunk, LOOP ? 4 : 5) surface_chunk_kernel(ItemDev one, ItemMap M) {
     __shared__ ChunkShared S;
     __shared__ BlockJob J;
     uint32_t local = blockIdx.x;
@@ -194,59 +202,126 @@ __global__ void __launch_bounds__(kChunk) surface_chunk_kernel(ItemDev one, Item
     uint32_t* __restrict__ table = it.table;
     const int layout = it.layout;
     const uint64_t n = info.n_triangles;
-    const uint64_t first = (uint64_t)local * kChunk;
-    if (first >= n) return;
-    const int count = (int)((n - first) < (uint64_t)kChunk ? (n - first) : (uint64_t)kChunk);
+    const uint32_t parts = it.parts > 1u ? it.parts : 1u;
+    const uint32_t span = (!LOOP || parts > 1u) ? 1u : (it.span > 1u ? it.span : 1u);
+    const uint32_t part = local % parts;
+    const uint64_t begin = (uint64_t)(local / parts) * span * kChunk;
+    if (begin >= n) return;
+    const uint64_t end = n - begin > (uint64_t)span * kChunk ? begin + (uint64_t)span * kChunk : n;
+    const uint32_t G = info.gridsize[0];
+    const uint64_t words = ((uint64_t)G * G * G + 31) / 32;
+    const bool dense = words <= (uint64_t)kHashCap;
     for (int i = threadIdx.x; i < kHashCap; i += kChunk) {
-        S.keys[i] = kEmpty;
+        S.keys[i] = dense && (uint64_t)i < words ? (uint32_t)i : kEmpty;
         S.vals[i] = 0u;
     }
-    if (threadIdx.x == 0) S.qn = 0;
-    const float* src = it.tris + first * 9;
-    for (int i = threadIdx.x; i < count * 9; i += kChunk) S.tri[i] = src[i];
-    __syncthreads();
-    if ((int)threadIdx.x < count) {
-        float v[3][3];
-        staged_vertices(S, threadIdx.x, info, v);
+    uint64_t first = begin;
+    do {
+        const int count = (int)(end - first < (uint64_t)kChunk ? end - first : (uint64_t)kChunk);
+        const float* src = it.tris + first * 9;
+        for (int i = threadIdx.x; i < count * 9; i += kChunk) S.tri[i] = src[i];
+        __syncthreads();
+        bool big = false;
         SurfTri T;
-        surface_setup(v, info, T);
-        const int nx = T.hi[0] - T.lo[0] + 1;
-        const int64_t vol = (int64_t)nx * (T.hi[1] - T.lo[1] + 1) * (T.hi[2] - T.lo[2] + 1);
-        bool queued = false;
-        if (vol > kBigVox || nx > 32) {
-            S.queue[atomicAdd(&S.qn, 1u)] = threadIdx.x;  // at most kChunk entries
-            queued = true;
+        if ((int)threadIdx.x < count) {
+            float v[3][3];
+            staged_vertices(S, threadIdx.x, info, v);
+            surface_box(v, info, T.lo, T.hi);
+            const int nx = T.hi[0] - T.lo[0] + 1;
+            const int64_t vol = (int64_t)nx * (T.hi[1] - T.lo[1] + 1) * (T.hi[2] - T.lo[2] + 1);
+            big = vol > kBigVox || nx > 32;
         }
-        if (!queued) lane_raster(S, table, T, info, layout);  // nx <= 32 here
-    }
-    __syncthreads();
-    // queued triangles: rows x 32-voxel chunks split over the workgroup
-    const unsigned nq = S.qn < (unsigned)kQueue ? S.qn : (unsigned)kQueue;
-    for (unsigned q = 0; q < nq; q++) {
-        float v[3][3];
-        staged_vertices(S, (int)S.queue[q], info, v);
-        SurfTri T;
-        surface_setup(v, info, T);
-        float xy[9], zx[9];
-        tri_coeffs(T, xy, zx);
-        const int ny = T.hi[1] - T.lo[1] + 1, nz = T.hi[2] - T.lo[2] + 1;
-        const int c0 = T.lo[0] >> 5, nch = (T.hi[0] >> 5) - c0 + 1;
-        const int64_t items = (int64_t)ny * nz * nch;
-        for (int64_t i = threadIdx.x; i < items; i += kChunk) {
-            const int ch = (int)(i % nch);
-            const int64_t r = i / nch;
-            const int y = T.lo[1] + (int)(r % ny), z = T.lo[2] + (int)(r / ny);
-            const float py = (float)y * info.unit[1], pz = (float)z * info.unit[2];
-            if (!edges_pass(T.yz, py, pz)) continue;
-            const int cx = (c0 + ch) << 5;
-            const int x0 = cx > T.lo[0] ? cx : T.lo[0], x1 = cx + 31 < T.hi[0] ? cx + 31 : T.hi[0];
-            chunk_row(S, table, T, xy, zx, info, layout, x0, x1, cx, y, z, py, pz);
+        const unsigned long long bal = __ballot(big);
+        const int wave = (int)(threadIdx.x >> 6);
+        if (__lane_id() == 0) S.wq[wave] = (unsigned)__popcll(bal);
+        if (part == 0 && (int)threadIdx.x < count && !big) {
+            float v[3][3];
+            staged_vertices(S, threadIdx.x, info, v);
+            surface_setup_tests(v, info, T);  // the box is T's already
+            lane_raster(S, table, dense, T, info, layout);  // nx <= 32 here
         }
-    }
-    __syncthreads();  // every hash insert is done
+        __syncthreads();  // S.tri is dead from here on; S.wq is complete
+        int myq = -1, nq = 0;  // this lane's queue slot (lane order), queue length
+#pragma unroll
+        for (int w = 0; w < kChunk / 64; w++) {
+            if (w == wave) myq = nq + (int)__popcll(bal & ((1ull << __lane_id()) - 1ull));
+            nq += (int)S.wq[w];
+        }
+        if (!big) myq = -1;
+        const float ux = info.unit[0], uy = info.unit[1], uz = info.unit[2];
+        for (int r0 = 0; r0 < nq; r0 += kRecs) {
+            const int nr = nq - r0 < kRecs ? nq - r0 : kRecs;
+            if (myq >= r0 && myq < r0 + nr) {
+                // the setup again, from memory (cheaper than keeping it in registers through the lane rasters)
+                float v[3][3];
+                load_tri(it.tris, first + threadIdx.x, info, v);
+                SurfTri T;
+                surface_setup(v, info, T);
+                const int xa = T.lo[0], xb = T.hi[0], ya = T.lo[1], yb = T.hi[1], za = T.lo[2], zb = T.hi[2];
+                int nk = 1, band = 0;
+                float sx = 0.0f;
+                band_chunks(T, info, xa, xb, yb, zb, nk, band, sx);
+                publish(S.recs[myq - r0], T, xa, xb, ya, yb, za, zb, nk, band, sx);
+                S.incl[myq - r0] = (xa > xb || ya > yb || za > zb)
+                                       ? 0ull
+                                       : (unsigned long long)(yb - ya + 1) * (unsigned long long)(zb - za + 1) *
+                                             (unsigned long long)nk;
+            }
+            __syncthreads();
+            if (threadIdx.x < 64) {  // inclusive scan of the round's counts (nr <= 64)
+                const int lane = (int)threadIdx.x;
+                unsigned long long c = lane < nr ? S.incl[lane] : 0ull;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const unsigned long long o = __shfl_up(c, d);
+                    if (lane >= d) c += o;
+                }
+                if (lane < nr) S.incl[lane] = c;
+            }
+            __syncthreads();
+            const unsigned long long total = S.incl[nr - 1];
+            for (unsigned long long j = threadIdx.x + (unsigned long long)part * kChunk; j < total;
+                 j += (unsigned long long)kChunk * parts) {
+                int lo = 0, hi = nr - 1;  // record: first whose inclusive prefix exceeds j
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (S.incl[mid] > j) hi = mid; else lo = mid + 1;
+                }
+                const SurfRec& R = S.recs[lo];
+                const unsigned long long q = j - (lo ? S.incl[lo - 1] : 0ull);
+                const unsigned nk = (unsigned)R.nk, ny = (unsigned)(R.yb - R.ya + 1);
+                int k, y, z;
+                if (q < 0x100000000ull) {  // 32-bit division whenever it suffices
+                    const unsigned row = (unsigned)q / nk;
+                    k = (int)((unsigned)q - row * nk);
+                    y = R.ya + (int)(row % ny);
+                    z = R.za + (int)(row / ny);
+                } else {
+                    const unsigned long long row = q / nk;
+                    k = (int)(q - row * nk);
+                    y = R.ya + (int)(row % ny);
+                    z = R.za + (int)(row / ny);
+                }
+                const float py = (float)y * uy, pz = (float)z * uz;
+                if (!pass3(R.yz, py, pz)) continue;
+                int bx0 = R.xa, bx1 = R.xb;
+                if (R.band) {
+                    plane_band(R, ux, py, pz, bx0, bx1);
+                    if (bx0 > bx1) continue;
+                }
+                const int cx = ((bx0 >> 5) + k) << 5;
+                if (cx > bx1) continue;
+                const int x0 = cx > bx0 ? cx : bx0, x1 = (cx + 31) < bx1 ? (cx + 31) : bx1;
+                emit_mask(S, table, dense, info, layout, cx, y, z, row_mask(R.n, R.d1, R.d2, R.xy, R.zx, ux, py, pz, x0, x1, cx));
+            }
+            __syncthreads();  // the records are rewritten by the next round
+        }
+        __syncthreads();  // S.wq is read above and rewritten by the next chunk
+        first += kChunk;
+    } while (LOOP && first < end);
     for (int i = threadIdx.x; i < kHashCap; i += kChunk) {
-        const uint32_t k = S.keys[i];
-        if (k != kEmpty) atomicOr(table + k, S.vals[i]);
+        const uint32_t k = S.keys[i], v = S.vals[i];
+        if (k != kEmpty && v != 0u) atomicOr(table + k, v);
     }
 }
 
@@ -281,7 +356,34 @@ uint32_t fill_units(uint32_t G) {
     return (uint32_t)((words + kFillWords - 1) / kFillWords);
 }
 
-uint32_t chunk_units(uint64_t n_triangles) { return (uint32_t)((n_triangles + kChunk - 1) / kChunk); }
+// Chunks per workgroup.  A table of at most kHashCap words is merged densely
+// in LDS, and every workgroup that touches it flushes up to all its words to
+// the same few addresses: such jobs give each workgroup several chunks (while
+// keeping at least 64 workgroups per job) to cut those atomics.
+uint32_t chunk_span(uint32_t G, uint64_t n_triangles) {
+    const uint64_t words = ((uint64_t)G * G * G + 31) / 32;
+    if (words > (uint64_t)kHashCap) return 1u;
+    const uint64_t chunks = (n_triangles + kChunk - 1) / kChunk;
+    const uint64_t s = chunks / 64;
+    return s < 1 ? 1u : (s > 16 ? 16u : (uint32_t)s);
+}
+
+// Workgroups per chunk.  A coarse mesh (fewer than G^2/4 triangles) has few
+// chunks of large triangles: its queued items are split over up to 32
+// workgroups per chunk, aiming at 256 workgroups per job.
+uint32_t chunk_parts(uint32_t G, uint64_t n_triangles) {
+    if (!prefer_tile_surface(n_triangles, G)) return 1u;
+    const uint64_t chunks = (n_triangles + kChunk - 1) / kChunk;
+    const uint64_t p = chunks ? 256 / chunks : 1;
+    return p < 1 ? 1u : (p > 32 ? 32u : (uint32_t)p);
+}
+
+uint32_t chunk_units(uint64_t n_triangles, uint32_t span, uint32_t parts) {
+    const uint64_t chunks = (n_triangles + kChunk - 1) / kChunk;
+    if (parts > 1u) return (uint32_t)(chunks * parts);
+    const uint64_t sp = span > 1u ? span : 1u;
+    return (uint32_t)((chunks + sp - 1) / sp);
+}
 
 hipError_t launch_fill_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, hipStream_t stream) {
     if (units == 0) return hipSuccess;
@@ -289,12 +391,16 @@ hipError_t launch_fill_jobs(const ItemDev& one, const ItemMap& M, uint32_t units
     return hipGetLastError();
 }
 
-hipError_t launch_chunk_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, hipStream_t stream) {
+hipError_t launch_chunk_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, bool loop, hipStream_t stream) {
     if (units == 0) return hipSuccess;
-    if (M.items)
-        surface_chunk_kernel<true><<<units, kChunk, 0, stream>>>(one, M);
+    if (M.items && loop)
+        surface_chunk_kernel<true, true><<<units, kChunk, 0, stream>>>(one, M);
+    else if (M.items)
+        surface_chunk_kernel<true, false><<<units, kChunk, 0, stream>>>(one, M);
+    else if (loop)
+        surface_chunk_kernel<false, true><<<units, kChunk, 0, stream>>>(one, M);
     else
-        surface_chunk_kernel<false><<<units, kChunk, 0, stream>>>(one, M);
+        surface_chunk_kernel<false, false><<<units, kChunk, 0, stream>>>(one, M);
     return hipGetLastError();
 }
 
@@ -304,6 +410,8 @@ static ItemDev single_job(const vox_info& info, const float* tris, uint32_t* tab
     one.tris = tris;
     one.table = table;
     one.layout = layout;
+    one.span = chunk_span(info.gridsize[0], info.n_triangles);
+    one.parts = chunk_parts(info.gridsize[0], info.n_triangles);
     return one;
 }
 
@@ -316,7 +424,9 @@ hipError_t launch_surface_chunk(const vox_info& info, const float* tris, uint32_
                                 hipStream_t stream) {
     if (info.n_triangles == 0) return hipSuccess;
     const ItemMap none{nullptr, nullptr, 1, 0};
-    return launch_chunk_jobs(single_job(info, tris, table, layout), none, chunk_units(info.n_triangles), stream);
+    const ItemDev one = single_job(info, tris, table, layout);
+    return launch_chunk_jobs(one, none, chunk_units(info.n_triangles, one.span, one.parts),
+                             one.span > 1u && one.parts <= 1u, stream);
 }
 
 }  // namespace voxamd

@@ -25,6 +25,7 @@
 //
 // G % 32 == 0 (rows are whole words); Morton: power-of-two G up to 4096.
 // Per-column arithmetic: vox_device.h (the reference CPU path bit for bit).
+#include "jobs.h"
 #include "vox_device.h"
 #include "vox_host.h"
 #include "vox_kernels.h"
@@ -41,11 +42,14 @@ constexpr int kMBlock = 256;
 constexpr int kMQueue = kMBlock;   // triangles with many columns, split over the workgroup
 constexpr int kMBigCols = 64;      // columns above which a triangle is queued
 constexpr uint32_t kSpillCap = 1u << 16;
+constexpr int kJobShift = 42;  // spill records: job index above the bit index (G^3 <= 2^42)
+constexpr unsigned long long kBitMask = (1ull << kJobShift) - 1ull;
+constexpr uint32_t kSuffixRows = 256;  // rows per work unit of the job-mapped suffix
 
 // misc words: [0] spill count, [1] spill overflow flag
 struct MarkerWs {
     unsigned* misc;
-    unsigned long long* spill;  // [2*i] first bit, [2*i+1] last bit (inclusive)
+    unsigned long long* spill;  // [2*i] job << kJobShift | first bit, [2*i+1] last bit (inclusive)
 };
 
 __global__ void __launch_bounds__(64) marker_init_kernel(unsigned* misc) {
@@ -55,7 +59,7 @@ __global__ void __launch_bounds__(64) marker_init_kernel(unsigned* misc) {
 // One hit column: marker toggle (and spill record) of the run x = 0..xmax.
 __device__ __forceinline__ void column_run(uint32_t* __restrict__ table, int64_t xmax, int y, int z, uint32_t G,
                                            uint64_t total_bits, const MarkerWs& ws, unsigned long long& offrow,
-                                           bool spills) {
+                                           bool spills, unsigned long long jobtag) {
     if (xmax < 0 || xmax >= (int64_t)G) offrow++;
     if (xmax < 0) return;
     const uint64_t row = (uint64_t)y * G + (uint64_t)z * G * G;
@@ -67,7 +71,7 @@ __device__ __forceinline__ void column_run(uint32_t* __restrict__ table, int64_t
         const uint64_t last = (uint64_t)xmax < total_bits - 1 - row ? row + (uint64_t)xmax : total_bits - 1;
         const unsigned slot = atomicAdd(ws.misc, 1u);
         if (slot < kSpillCap) {
-            ws.spill[2 * slot] = row + G;
+            ws.spill[2 * slot] = jobtag | (row + G);
             ws.spill[2 * slot + 1] = last;
         } else {
             ws.misc[1] = 1u;
@@ -75,15 +79,32 @@ __device__ __forceinline__ void column_run(uint32_t* __restrict__ table, int64_t
     }
 }
 
-__global__ void __launch_bounds__(kMBlock) solid_marker_kernel(vox_info info, const float* __restrict__ tris,
-                                                                uint32_t* __restrict__ table, MarkerWs ws,
-                                                                unsigned long long* __restrict__ offrow_out,
-                                                                int spills) {
+// Work unit: kMBlock consecutive triangles of one job; the markers go to the
+// job's table.  MULTI: the job comes from the launch's job map (block
+// prefix); otherwise it is `one`.
+template <bool MULTI>
+__global__ void __launch_bounds__(kMBlock) solid_marker_kernel(ItemDev one, ItemMap M, MarkerWs ws, int spills) {
     __shared__ float s_tri[kMBlock * 9];
     __shared__ unsigned queue[kMQueue];
     __shared__ unsigned qn;
+    __shared__ BlockJob J;
+    uint32_t local = blockIdx.x;
+    int job = 0;
+    if (MULTI) {
+        resolve_job(one, M, blockIdx.x, J);
+        local = J.local;
+        job = J.index;
+        if (local == ~0u) return;
+    }
+    const ItemDev& it = MULTI ? J.it : one;
+    const vox_info& info = it.info;
+    const float* __restrict__ tris = it.tris;
+    uint32_t* __restrict__ table = it.table;
+    unsigned long long* __restrict__ offrow_out = it.offrow;
+    const unsigned long long jobtag = (unsigned long long)job << kJobShift;
     const uint64_t n = info.n_triangles;
-    const uint64_t first = (uint64_t)blockIdx.x * kMBlock;
+    const uint64_t first = (uint64_t)local * kMBlock;
+    if (first >= n) return;
     const int count = (int)((n - first) < (uint64_t)kMBlock ? (n - first) : (uint64_t)kMBlock);
     const int G = (int)info.gridsize[0];
     const uint64_t total_bits = (uint64_t)G * G * G;
@@ -127,7 +148,8 @@ __global__ void __launch_bounds__(kMBlock) solid_marker_kernel(vox_info info, co
                     for (int z = zlo; z <= zhi; z++) {
                         const float pz = ((float)z + 0.5f) * info.unit[2];
                         if (!column_hit(S, py, pz)) continue;
-                        column_run(table, column_xmax(S, info, py, pz), y, z, (uint32_t)G, total_bits, ws, offrow, spills != 0);
+                        column_run(table, column_xmax(S, info, py, pz), y, z, (uint32_t)G, total_bits, ws, offrow,
+                                   spills != 0, jobtag);
                     }
                 }
             }
@@ -156,7 +178,8 @@ __global__ void __launch_bounds__(kMBlock) solid_marker_kernel(vox_info info, co
             const float py = ((float)y + 0.5f) * info.unit[1];
             const float pz = ((float)z + 0.5f) * info.unit[2];
             if (!column_hit(S, py, pz)) continue;
-            column_run(table, column_xmax(S, info, py, pz), y, z, (uint32_t)G, total_bits, ws, offrow, spills != 0);
+            column_run(table, column_xmax(S, info, py, pz), y, z, (uint32_t)G, total_bits, ws, offrow,
+                       spills != 0, jobtag);
         }
     }
     if (offrow_out && offrow) atomicAdd(offrow_out, offrow);
@@ -172,22 +195,21 @@ __device__ __forceinline__ uint32_t prefix_xor_up(uint32_t w) {
     return w;
 }
 
-// Rows of rw = G/32 words.  rw <= 64: a wave holds 64/rw whole rows (lane =
+// Rows r_begin..r_end-1 of rw = G/32 words, in place; waves `wave` of
+// `nwaves` stride over them.  rw <= 64: a wave holds 64/rw whole rows (lane =
 // word); rw > 64: a wave walks its row from the last 64-word segment down,
 // carrying the parity.
-__global__ void __launch_bounds__(256) suffix_rows_kernel(uint32_t* __restrict__ table, uint32_t G,
-                                                          uint64_t nrows) {
+__device__ __forceinline__ void suffix_rows_range(uint32_t* __restrict__ table, uint32_t G, uint64_t r_begin,
+                                                  uint64_t r_end, uint64_t wave, uint64_t nwaves) {
     const uint32_t rw = G >> 5;
     const int lane = __lane_id();
-    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     if (rw <= 64) {
         const uint32_t per = 64 / rw;             // rows per wave step
         const uint32_t r_in = (uint32_t)lane / rw, j = (uint32_t)lane % rw;
         const bool active_lane = r_in < per;
-        for (uint64_t r0 = wave * per; r0 < nrows; r0 += nwaves * per) {
+        for (uint64_t r0 = r_begin + wave * per; r0 < r_end; r0 += nwaves * per) {
             const uint64_t r = r0 + r_in;
-            const bool act = active_lane && r < nrows;
+            const bool act = active_lane && r < r_end;
             const uint64_t idx = r * rw + j;
             const uint32_t w = act ? table[idx] : 0u;
             const unsigned long long par = __ballot(act && (__popc(w) & 1));
@@ -200,7 +222,7 @@ __global__ void __launch_bounds__(256) suffix_rows_kernel(uint32_t* __restrict__
             if (act) table[idx] = prefix_xor_up(w) ^ (carry ? 0xFFFFFFFFu : 0u);
         }
     } else {
-        for (uint64_t r = wave; r < nrows; r += nwaves) {
+        for (uint64_t r = r_begin + wave; r < r_end; r += nwaves) {
             uint32_t carry = 0;  // parity of the words of larger x already done
             for (int64_t hi = (int64_t)rw; hi > 0; hi -= 64) {
                 const int64_t lo = hi > 64 ? hi - 64 : 0;
@@ -217,21 +239,20 @@ __global__ void __launch_bounds__(256) suffix_rows_kernel(uint32_t* __restrict__
     }
 }
 
-// In-place suffix for G % 128 == 0: a lane holds 4 consecutive words (one
-// 16-byte load), a row is rw/4 <= 64 lanes, a wave holds 256/rw rows.
-__global__ void __launch_bounds__(256) suffix_rows4_kernel(uint32_t* __restrict__ table, uint32_t G,
-                                                           uint64_t nrows) {
+// The same for G % 128 == 0 and a 16-byte aligned table: a lane holds 4
+// consecutive words (one 16-byte load), a row is rw/4 <= 64 lanes, a wave
+// holds 256/rw rows.
+__device__ __forceinline__ void suffix_rows4_range(uint32_t* __restrict__ table, uint32_t G, uint64_t r_begin,
+                                                   uint64_t r_end, uint64_t wave, uint64_t nwaves) {
     const uint32_t lpr = G >> 7;  // lanes per row
     const uint32_t per = 64 / lpr;
     const int lane = __lane_id();
     const uint32_t r_in = (uint32_t)lane / lpr, j = (uint32_t)lane % lpr;
     const bool active_lane = r_in < per;
-    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     uint4* t4 = (uint4*)table;
-    for (uint64_t r0 = wave * per; r0 < nrows; r0 += nwaves * per) {
+    for (uint64_t r0 = r_begin + wave * per; r0 < r_end; r0 += nwaves * per) {
         const uint64_t r = r0 + r_in;
-        const bool act = active_lane && r < nrows;
+        const bool act = active_lane && r < r_end;
         const uint64_t idx = r * lpr + j;
         uint4 v = act ? t4[idx] : make_uint4(0u, 0u, 0u, 0u);
         const uint32_t pa = __popc(v.x) & 1, pb = __popc(v.y) & 1, pc = __popc(v.z) & 1, pd = __popc(v.w) & 1;
@@ -247,6 +268,39 @@ __global__ void __launch_bounds__(256) suffix_rows4_kernel(uint32_t* __restrict_
         v.x = prefix_xor_up(v.x) ^ (0u - cx);
         if (act) t4[idx] = v;
     }
+}
+
+__global__ void __launch_bounds__(256) suffix_rows_kernel(uint32_t* __restrict__ table, uint32_t G,
+                                                          uint64_t nrows) {
+    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    suffix_rows_range(table, G, 0, nrows, wave, nwaves);
+}
+
+__global__ void __launch_bounds__(256) suffix_rows4_kernel(uint32_t* __restrict__ table, uint32_t G,
+                                                           uint64_t nrows) {
+    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    suffix_rows4_range(table, G, 0, nrows, wave, nwaves);
+}
+
+// Batched form: work unit = kSuffixRows rows of one job's (linear, in-place)
+// table.
+__global__ void __launch_bounds__(256) suffix_jobs_kernel(ItemMap M) {
+    __shared__ BlockJob J;
+    const ItemDev none{};
+    resolve_job(none, M, blockIdx.x, J);
+    if (J.local == ~0u) return;
+    const uint32_t G = J.it.info.gridsize[0];
+    uint32_t* __restrict__ table = J.it.table;
+    const uint64_t nrows = (uint64_t)G * G;
+    const uint64_t r0 = (uint64_t)J.local * kSuffixRows;
+    const uint64_t r1 = r0 + kSuffixRows < nrows ? r0 + kSuffixRows : nrows;
+    const uint64_t wave = threadIdx.x >> 6;
+    if ((G & 127u) == 0 && (((uintptr_t)table) & 15u) == 0)
+        suffix_rows4_range(table, G, r0, r1, wave, 4);
+    else
+        suffix_rows_range(table, G, r0, r1, wave, 4);
 }
 
 // Rows of the linear marker table -> output table, one workgroup per block
@@ -402,12 +456,15 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
     }
 }
 
-// Spilled run parts, XORed into the final table; one workgroup per record.
-__global__ void __launch_bounds__(256) spill_apply_kernel(uint32_t* __restrict__ table, MarkerWs ws) {
+// Spilled run parts, XORed into the final tables (the record names the job of
+// `M`, or `one`); one workgroup per record.
+__global__ void __launch_bounds__(256) spill_apply_kernel(ItemDev one, ItemMap M, MarkerWs ws) {
     if (ws.misc[1] != 0u) return;  // overflowed: spill_overflow_kernel applies every spill
     const unsigned ns = ws.misc[0] < kSpillCap ? ws.misc[0] : kSpillCap;
     for (unsigned s = blockIdx.x; s < ns; s += gridDim.x) {
-        const uint64_t b0 = ws.spill[2 * s], b1 = ws.spill[2 * s + 1];
+        const unsigned long long rec = ws.spill[2 * s];
+        const uint64_t b0 = rec & kBitMask, b1 = ws.spill[2 * s + 1];
+        uint32_t* __restrict__ table = job_ref(one, M, (int)(rec >> kJobShift)).table;
         for (uint64_t w = (b0 >> 5) + threadIdx.x; w <= (b1 >> 5); w += blockDim.x) {
             const uint32_t lo = w == (b0 >> 5) ? (uint32_t)(b0 & 31u) : 0u;
             const uint32_t hi = w == (b1 >> 5) ? (uint32_t)(b1 & 31u) : 31u;
@@ -418,15 +475,21 @@ __global__ void __launch_bounds__(256) spill_apply_kernel(uint32_t* __restrict__
 
 // Spill list overflow (more than kSpillCap spilled runs): every triangle's
 // spills are recomputed and applied directly.  Exits at once otherwise.
-__global__ void __launch_bounds__(256) spill_overflow_kernel(vox_info info, const float* __restrict__ tris,
-                                                             uint32_t* __restrict__ table, MarkerWs ws) {
+// Triangles: all of `one`, or of every job of `M` (base = triangle prefix).
+__global__ void __launch_bounds__(256) spill_overflow_kernel(ItemDev one, ItemMap M, MarkerWs ws) {
     if (ws.misc[1] == 0u) return;
-    const int G = (int)info.gridsize[0];
-    const uint64_t total_bits = (uint64_t)G * G * G;
-    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < info.n_triangles;
-         t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t total = M.items ? (uint64_t)M.base[M.nitems] : one.info.n_triangles;
+    for (uint64_t gt = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; gt < total;
+         gt += (uint64_t)gridDim.x * blockDim.x) {
+        const int j = M.items ? job_of_triangle(M, (uint32_t)gt) : 0;
+        const ItemDev& it = job_ref(one, M, j);
+        const vox_info& info = it.info;
+        uint32_t* __restrict__ table = it.table;
+        const uint64_t t = M.items ? gt - M.base[j] : gt;
+        const int G = (int)info.gridsize[0];
+        const uint64_t total_bits = (uint64_t)G * G * G;
         float v[3][3];
-        load_tri(tris, t, info, v);
+        load_tri(it.tris, t, info, v);
         SolidTri S;
         solid_setup(v, info, S);
         if (!S.valid) continue;
@@ -464,7 +527,7 @@ static std::map<std::pair<int, hipStream_t>, MarkerBuffers> g_mws;
 
 bool marker_supported(uint32_t G, int layout, bool accumulate) {
     (void)accumulate;
-    if (G < 32 || (G & 31u)) return false;
+    if (G < 32 || (G & 31u) || G > 16384u) return false;  // bit indices below 2^42 (spill records)
     if (layout == VOX_LAYOUT_MORTON) return (G & (G - 1)) == 0 && G <= 4096;
     return true;
 }
@@ -505,10 +568,19 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
     e = launch_fill_zero(info, markers, stream);
     if (e != hipSuccess) return e;
     const bool linear = layout == VOX_LAYOUT_LINEAR;
+    const ItemMap none{nullptr, nullptr, 1, 0};
+    ItemDev one{};  // markers: into `markers`
+    one.info = info;
+    one.tris = tris;
+    one.table = markers;
+    one.offrow = offrow;
+    one.layout = layout;
+    one.solid = 1;
+    ItemDev fin = one;  // spills: into the final table
+    fin.table = table;
     if (info.n_triangles) {
         const uint64_t blocks = (info.n_triangles + kMBlock - 1) / kMBlock;
-        solid_marker_kernel<<<(unsigned)blocks, kMBlock, 0, stream>>>(info, tris, markers, ws, offrow,
-                                                                       linear ? 1 : 0);
+        solid_marker_kernel<false><<<(unsigned)blocks, kMBlock, 0, stream>>>(one, none, ws, linear ? 1 : 0);
     }
     if (in_place && (G & 127u) == 0 && (((uintptr_t)table) & 15u) == 0) {
         const uint64_t nrows = (uint64_t)G * G;
@@ -531,9 +603,31 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
                                                                                  accumulate ? 1 : 0);
     }
     if (linear) {
-        spill_apply_kernel<<<64, 256, 0, stream>>>(table, ws);
-        spill_overflow_kernel<<<256, 256, 0, stream>>>(info, tris, table, ws);
+        spill_apply_kernel<<<64, 256, 0, stream>>>(fin, none, ws);
+        spill_overflow_kernel<<<256, 256, 0, stream>>>(fin, none, ws);
     }
+    return hipGetLastError();
+}
+
+// ---- batched form (linear layout, no accumulation: in place) -------------------
+uint64_t marker_ws_bytes() { return 256 + 16ull * kSpillCap; }
+uint32_t marker_block_units(uint64_t n_triangles) { return (uint32_t)((n_triangles + kMBlock - 1) / kMBlock); }
+uint32_t marker_row_units(uint32_t G) { return (uint32_t)(((uint64_t)G * G + kSuffixRows - 1) / kSuffixRows); }
+bool marker_batch_supported(uint32_t G, int layout, bool accumulate) {
+    return layout == VOX_LAYOUT_LINEAR && !accumulate && marker_supported(G, layout, accumulate);
+}
+
+// The tables are zero-filled beforehand (the batch's fill class).  m_blocks:
+// block prefix per job, m_rows: suffix unit prefix, m_tris: triangle prefix.
+hipError_t launch_marker_jobs(const ItemMap& m_blocks, const ItemMap& m_rows, const ItemMap& m_tris,
+                              uint32_t blocks, uint32_t row_units, void* mws, hipStream_t stream) {
+    const MarkerWs ws{(unsigned*)mws, (unsigned long long*)((char*)mws + 256)};
+    const ItemDev dummy{};
+    marker_init_kernel<<<1, 64, 0, stream>>>(ws.misc);
+    if (blocks) solid_marker_kernel<true><<<blocks, kMBlock, 0, stream>>>(dummy, m_blocks, ws, 1);
+    if (row_units) suffix_jobs_kernel<<<row_units, 256, 0, stream>>>(m_rows);
+    spill_apply_kernel<<<64, 256, 0, stream>>>(dummy, m_blocks, ws);
+    spill_overflow_kernel<<<256, 256, 0, stream>>>(dummy, m_tris, ws);
     return hipGetLastError();
 }
 

@@ -42,6 +42,8 @@ struct ItemDev {
     int32_t layout, solid, accumulate;
     uint32_t tile_base;  // first global tile id of this job (tile algorithm)
     uint64_t tri_base;   // first global triangle id of this job
+    uint32_t span;       // chunk algorithm: chunks per workgroup (0 or 1: one)
+    uint32_t parts;      // chunk algorithm: workgroups per chunk (0 or 1: one)
 };
 
 // Work mapping of one launch: a single job passed by value (items == nullptr)

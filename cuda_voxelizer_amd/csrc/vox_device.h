@@ -96,7 +96,20 @@ __device__ __forceinline__ void edge_test(EdgeTest& E, const float v[3][3], cons
     }
 }
 
-__device__ __forceinline__ void surface_setup(const float v[3][3], const vox_info& info, SurfTri& T) {
+// Clamped voxel box of a translated triangle (cpu_voxelizer.cpp:71-80).
+__device__ __forceinline__ void surface_box(const float v[3][3], const vox_info& info, int lo[3], int hi[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int gmax = (int)info.gridsize[k] - 1;
+        float mn = hmin(v[0][k], hmin(v[1][k], v[2][k]));
+        float mx = hmax(v[0][k], hmax(v[1][k], v[2][k]));
+        lo[k] = iclamp((int)(mn / info.unit[k]), 0, gmax);
+        hi[k] = iclamp((int)(mx / info.unit[k]), 0, gmax);
+    }
+}
+
+// Everything of surface_setup except the box (T.lo / T.hi are left alone).
+__device__ __forceinline__ void surface_setup_tests(const float v[3][3], const vox_info& info, SurfTri& T) {
     float e[3][3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -105,14 +118,6 @@ __device__ __forceinline__ void surface_setup(const float v[3][3], const vox_inf
         e[2][k] = v[0][k] - v[2][k];
     }
     unit_normal(e[0], e[1], T.n);
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        const int gmax = (int)info.gridsize[k] - 1;
-        float mn = hmin(v[0][k], hmin(v[1][k], v[2][k]));
-        float mx = hmax(v[0][k], hmax(v[1][k], v[2][k]));
-        T.lo[k] = iclamp((int)(mn / info.unit[k]), 0, gmax);
-        T.hi[k] = iclamp((int)(mx / info.unit[k]), 0, gmax);
-    }
     // critical point c and the two plane offsets (cpu_voxelizer.cpp:83-87)
     float c[3];
 #pragma unroll
@@ -128,6 +133,11 @@ __device__ __forceinline__ void surface_setup(const float v[3][3], const vox_inf
     edge_test(T.xy, v, e, 0, 1, T.n[2] < 0.0f, info.unit[0], info.unit[1]);
     edge_test(T.yz, v, e, 1, 2, T.n[0] < 0.0f, info.unit[1], info.unit[2]);
     edge_test(T.zx, v, e, 2, 0, T.n[1] < 0.0f, info.unit[0], info.unit[2]);
+}
+
+__device__ __forceinline__ void surface_setup(const float v[3][3], const vox_info& info, SurfTri& T) {
+    surface_box(v, info, T.lo, T.hi);
+    surface_setup_tests(v, info, T);
 }
 
 // Projection pass: all three edges of one projection accept point (pu, pv).

@@ -26,6 +26,12 @@ hipError_t launch_tiled(const vox_info& info, const float* tris, uint32_t* table
                         unsigned long long* offrow, hipStream_t stream);
 
 bool marker_supported(uint32_t G, int layout, bool accumulate);
+bool marker_batch_supported(uint32_t G, int layout, bool accumulate);
+uint64_t marker_ws_bytes();
+uint32_t marker_block_units(uint64_t n_triangles);
+uint32_t marker_row_units(uint32_t G);
+hipError_t launch_marker_jobs(const ItemMap& m_blocks, const ItemMap& m_rows, const ItemMap& m_tris,
+                              uint32_t blocks, uint32_t row_units, void* mws, hipStream_t stream);
 hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t* table, int layout,
                                bool accumulate, unsigned long long* offrow, hipStream_t stream);
 
@@ -43,9 +49,11 @@ inline bool prefer_tile_surface(uint64_t n_triangles, uint32_t G) {
 }
 hipError_t launch_fill_zero(const vox_info& info, uint32_t* table, hipStream_t stream);
 uint32_t fill_units(uint32_t G);
-uint32_t chunk_units(uint64_t n_triangles);
+uint32_t chunk_span(uint32_t G, uint64_t n_triangles);
+uint32_t chunk_parts(uint32_t G, uint64_t n_triangles);
+uint32_t chunk_units(uint64_t n_triangles, uint32_t span, uint32_t parts);
 hipError_t launch_fill_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, hipStream_t stream);
-hipError_t launch_chunk_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, hipStream_t stream);
+hipError_t launch_chunk_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, bool loop, hipStream_t stream);
 hipError_t launch_surface_chunk(const vox_info& info, const float* tris, uint32_t* table, int layout,
                                 hipStream_t stream);
 

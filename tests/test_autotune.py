@@ -13,7 +13,7 @@ def test_choose_times_once_per_class_and_persists(tmp_path):
 
     def timer(job, algo):
         calls.append((job.info.grid, algo))
-        return {"chunk": 3.0, "tile": 1.0, "direct": 2.0}[algo]
+        return {"chunk": 3.0, "marker": 4.0, "tile": 1.0, "direct": 2.0}[algo]
 
     path = str(tmp_path / "tune.json")
     t = Tuner(path, device_name="fake", timer=timer)
@@ -23,7 +23,7 @@ def test_choose_times_once_per_class_and_persists(tmp_path):
     # 64/surface/n2^9..10 are different buckets? 1000 -> 2^9, 1500 -> 2^10
     assert class_key(64, False, False, 1000) != class_key(64, False, False, 1500)
     n_first = len(calls)
-    assert all(a in ("chunk", "tile", "direct") for _, a in calls)
+    assert all(a in ("chunk", "marker", "tile", "direct") for _, a in calls)
     t2 = Tuner(path, device_name="fake", timer=timer)
     t2.tune_jobs([_job(64, False, 1000)])
     assert len(calls) == n_first  # served from the cache file
@@ -35,6 +35,37 @@ def test_choose_times_once_per_class_and_persists(tmp_path):
 def test_solid_jobs_never_get_chunk(tmp_path):
     from cuda_voxelizer_amd.autotune import Tuner
     t = Tuner(str(tmp_path / "t.json"), device_name="fake",
-              timer=lambda job, algo: {"chunk": 0.1, "tile": 1.0, "direct": 2.0}[algo])
+              timer=lambda job, algo: {"chunk": 0.1, "marker": 3.0, "tile": 1.0, "direct": 2.0}[algo])
     j = _job(32, True, 100)
     assert t.choose(j) == "tile"
+
+
+def test_copies_fill_the_device_within_the_table_budget(tmp_path):
+    import numpy as np
+    from cuda_voxelizer_amd.autotune import Tuner
+    t = Tuner(str(tmp_path / "t.json"), device_name="fake", timer=lambda job, algo: 1.0,
+              fill_triangles=2_000_000, max_table_bytes=64 << 20)
+
+    def job(grid, n):
+        words = (grid ** 3 + 31) // 32
+        return SimpleNamespace(info=SimpleNamespace(grid=grid, n_triangles=n),
+                               table=SimpleNamespace(numel=lambda: words, element_size=lambda: 4))
+
+    assert t.copies(job(64, 500)) == 64        # tiny mesh: the most copies
+    assert t.copies(job(64, 100_000)) == 20    # ceil(2M / 100k)
+    assert t.copies(job(64, 5_000_000)) == 4   # large mesh: at least 4
+    assert t.copies(job(256, 500)) == 32       # 2 MiB tables: 64 MiB budget
+    assert t.copies(job(1024, 500)) == 1       # one 128 MiB table exceeds the budget: still one copy
+    assert np.all([t.copies(job(g, n)) >= 1 for g in (1, 32, 512) for n in (0, 1, 10 ** 7)])
+
+
+def test_pick_rules_out_a_long_critical_path(tmp_path):
+    from cuda_voxelizer_amd.autotune import Tuner
+    t = Tuner(str(tmp_path / "t.json"), device_name="fake", timer=lambda job, algo: 1.0)
+    # direct: cheapest per copy, but one copy alone runs 1.3 ms (a straggler thread) -> ruled out
+    times = {"tile": (60e-6, 30e-6), "direct": (1.3e-3, 20e-6)}
+    assert t.pick(times) == "tile"
+    # within the slack the per-copy time decides
+    times = {"chunk": (80e-6, 10e-6), "tile": (50e-6, 40e-6), "direct": (300e-6, 5e-6)}
+    assert t.pick(times) == "chunk"
+    assert t.pick({"tile": 2.0, "direct": 1.0}) == "direct"  # plain numbers: both times

@@ -20,11 +20,11 @@ def _jobs(gpu, count=24, seed=0xB1, n_max=20000):
     from cuda_voxelizer_amd import meshgen
     meshes = meshgen.batch_meshes(count, seed=seed, n_min=50, n_max=n_max, grids=(16, 32, 33, 64, 96, 128))
     jobs, host = [], []
-    algos = ["auto", "tile", "chunk", "direct"]
+    algos = ["auto", "tile", "chunk", "direct", "marker"]
     for i, m in enumerate(meshes):
         morton = (m.grid & (m.grid - 1)) == 0 and i % 2 == 0
-        algo = algos[i % 4]
-        if m.solid and algo == "chunk":
+        algo = algos[i % 5]
+        if (m.solid and algo == "chunk") or (not m.solid and algo == "marker"):
             algo = "auto"
         info = V.voxinfo_for_mesh(m.tris.reshape(-1, 3), m.grid, m.tris.shape[0])
         d = torch.from_numpy(m.tris.reshape(-1)).to(gpu)
@@ -87,4 +87,45 @@ def test_batch_sees_new_triangle_contents(oracle, gpu):
         new_host.append(t2)
     b.launch()
     _check(oracle, jobs, new_host)
+    b.close()
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_marker_spills_per_job(oracle, gpu, use_graph):
+    """Solid jobs of the marker class whose runs leave their rows: the spill
+    records carry the job index, so each spill lands in its own job's table
+    (jobs of different grids, with and without spills, and offrow counts)."""
+    import torch
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import meshgen
+    jobs, host, cases = [], [], []
+    for k, grid in enumerate((32, 64, 96, 32)):
+        box = V.AABox(np.zeros(3, np.float32), np.full(3, grid, np.float32))
+        tris = []
+        for c in (grid + 1.7, 2.2 * grid, 0.2, 0.4 * grid, -0.7):
+            for yz in (np.array([[0.2, 0.3], [3.1, 0.6], [0.9, 3.4]]) + k,
+                       np.array([[0.2, 0.3], [grid - 0.9, 0.6], [0.9, grid - 0.6]]),
+                       np.array([[grid - 2.8, grid - 2.7], [grid - 0.1, grid - 2.4], [grid - 2.1, grid - 0.1]])):
+                tris.append(np.concatenate([np.full((3, 1), c), yz], 1))
+        tris = np.stack(tris).astype(np.float32)
+        cases.append((V.VoxInfo(box, grid, tris.shape[0]), tris))
+    v, f = meshgen.icosphere_sh(level=4, seed=3)  # a closed mesh without spills in the same class
+    sh = meshgen.to_aos(v, f)
+    cases.insert(2, (V.voxinfo_for_mesh(sh.reshape(-1, 3), 64, sh.shape[0]), sh))
+    for info, tris in cases:
+        d = torch.from_numpy(np.ascontiguousarray(tris).reshape(-1)).to(gpu)
+        t = torch.full((V.table_words(info.grid),), -1, dtype=torch.int32, device=gpu)
+        off = torch.zeros(1, dtype=torch.int64, device=gpu)
+        jobs.append(V.BatchJob(info, d, t, solid=True, algo="marker", offrow=off))
+        host.append(tris)
+    b = V.Batch(jobs, use_graph=use_graph)
+    for _ in range(2):  # the second launch reuses the spill workspace
+        for j in jobs:
+            j.table.fill_(-1)
+            j.offrow.zero_()
+        b.launch()
+        _check(oracle, jobs, host)
+        for j, tris in zip(jobs, host):
+            _, ocl, _ = oracle.solid(oracle.info_from_bytes(j.info.as_bytes()), tris)
+            assert int(j.offrow.item()) == ocl
     b.close()

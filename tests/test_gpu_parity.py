@@ -289,3 +289,29 @@ def test_solid_marker_scratch_reuse(oracle, gpu):
         if acc:
             exp = exp ^ base
         assert _mismatch(_host_table(t), exp) == 0, (grid, morton, acc)
+
+
+@pytest.mark.parametrize("grid", [256, 512])
+def test_large_triangle_soup_chunk(oracle, gpu, grid):
+    """Chunks whose triangles are all queued (boxes far above 256 voxels), so the
+    records go through several rounds, mixed with tiny and grid-spanning ones."""
+    from cuda_voxelizer_amd import meshgen
+    big = meshgen.triangle_soup(1500, seed=grid + 1, mu=-2.0, sigma=0.8)
+    tiny = meshgen.triangle_soup(600, seed=grid + 2, mu=-7.0, sigma=0.5)
+    span = np.array([[[0, 0, 0], [1, 1, 0.2], [0.1, 0.9, 1]], [[1, 0, 0], [0, 1, 1], [0.5, 0.5, 0]]], np.float32)
+    tris = np.concatenate([big[:700], tiny, span, big[700:]])
+    for morton in (False, True):
+        got, exp = _run(oracle, gpu, tris, grid, morton, False, "chunk")
+        assert _mismatch(got, exp) == 0, morton
+
+
+@pytest.mark.parametrize("grid", [8, 32, 40, 41])
+def test_small_table_dense_merge_chunk(oracle, gpu, grid):
+    """Tables of at most 2048 words are merged densely in LDS with several chunks
+    per workgroup (40^3 is the largest such grid, 41^3 the smallest hashed one)."""
+    from cuda_voxelizer_amd import meshgen
+    tris = np.concatenate([meshgen.torus_slivers(nu=256, nv=128, grid=grid, seed=grid),
+                           meshgen.triangle_soup(20000, seed=grid, mu=-3.0, sigma=1.0)])
+    for morton in ((False, True) if grid & (grid - 1) == 0 else (False,)):
+        got, exp = _run(oracle, gpu, tris, grid, morton, False, "chunk")
+        assert _mismatch(got, exp) == 0, morton
