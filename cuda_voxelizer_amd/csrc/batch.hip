@@ -262,7 +262,8 @@ vox_status vox_batch_create(const vox_batch_item* items, int count, unsigned fla
     }
     bind_maps(b, b->dev_mem);
     if (!b->marker.items.empty()) {
-        e = hipMalloc(&b->mws, marker_ws_bytes());  // counters are reset by a kernel at every launch
+        e = hipMalloc(&b->mws, marker_ws_bytes());
+        if (e == hipSuccess) e = hipMemset(b->mws, 0, marker_ws_bytes());  // counters start at zero
         if (e != hipSuccess) {
             free_batch(b);
             return fail(VOX_ERR_OOM, std::string("batch marker workspace: ") + hipGetErrorString(e));
@@ -393,7 +394,9 @@ vox_status vox_runner_launch(vox_runner* r, const vox_batch_item* items, int cou
     }
     if (!r->b.marker.items.empty() && !r->b.mws) {  // fixed size, allocated once
         e = hipMalloc(&r->b.mws, marker_ws_bytes());
+        if (e == hipSuccess) e = hipMemset(r->b.mws, 0, marker_ws_bytes());  // counters start at zero
         if (e != hipSuccess) {
+            if (r->b.mws) (void)hipFree(r->b.mws);
             r->b.mws = nullptr;
             return fail(VOX_ERR_OOM, std::string("runner marker workspace: ") + hipGetErrorString(e));
         }

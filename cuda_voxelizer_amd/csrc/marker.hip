@@ -46,15 +46,14 @@ constexpr int kJobShift = 42;  // spill records: job index above the bit index (
 constexpr unsigned long long kBitMask = (1ull << kJobShift) - 1ull;
 constexpr uint32_t kSuffixRows = 256;  // rows per work unit of the job-mapped suffix
 
-// misc words: [0] spill count, [1] spill overflow flag
+// misc words: [0] spill count, [1] spill overflow flag, [2] spill blocks done.
+// Zero when allocated; the spill pass returns them to zero, so every call
+// starts from zero without a reset launch (and the Morton layout, which
+// records no spills, never touches them).
 struct MarkerWs {
     unsigned* misc;
     unsigned long long* spill;  // [2*i] job << kJobShift | first bit, [2*i+1] last bit (inclusive)
 };
-
-__global__ void __launch_bounds__(64) marker_init_kernel(unsigned* misc) {
-    if (threadIdx.x < 4) misc[threadIdx.x] = 0u;
-}
 
 // One hit column: marker toggle (and spill record) of the run x = 0..xmax.
 __device__ __forceinline__ void column_run(uint32_t* __restrict__ table, int64_t xmax, int y, int z, uint32_t G,
@@ -456,62 +455,72 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
     }
 }
 
-// Spilled run parts, XORed into the final tables (the record names the job of
-// `M`, or `one`); one workgroup per record.
-__global__ void __launch_bounds__(256) spill_apply_kernel(ItemDev one, ItemMap M, MarkerWs ws) {
-    if (ws.misc[1] != 0u) return;  // overflowed: spill_overflow_kernel applies every spill
-    const unsigned ns = ws.misc[0] < kSpillCap ? ws.misc[0] : kSpillCap;
-    for (unsigned s = blockIdx.x; s < ns; s += gridDim.x) {
-        const unsigned long long rec = ws.spill[2 * s];
-        const uint64_t b0 = rec & kBitMask, b1 = ws.spill[2 * s + 1];
-        uint32_t* __restrict__ table = job_ref(one, M, (int)(rec >> kJobShift)).table;
-        for (uint64_t w = (b0 >> 5) + threadIdx.x; w <= (b1 >> 5); w += blockDim.x) {
-            const uint32_t lo = w == (b0 >> 5) ? (uint32_t)(b0 & 31u) : 0u;
-            const uint32_t hi = w == (b1 >> 5) ? (uint32_t)(b1 & 31u) : 31u;
-            atomicXor(table + w, (0xFFFFFFFFu >> lo) & (0xFFFFFFFFu << (31u - hi)));
+// Spilled run parts, XORed into the final tables: from the records (each
+// names its job of `M_jobs`, or `one`), one workgroup per record -- or, when
+// more than kSpillCap runs spilled, recomputed from every triangle (all of
+// `one`, or of every job of `M_tris`, whose base is the triangle prefix).
+// The last workgroup to finish returns the counters to zero.
+__global__ void __launch_bounds__(256) spill_kernel(ItemDev one, ItemMap M_jobs, ItemMap M_tris, MarkerWs ws) {
+    const unsigned spilled = ws.misc[0], overflow = ws.misc[1];
+    if (!overflow) {
+        const unsigned ns = spilled < kSpillCap ? spilled : kSpillCap;
+        for (unsigned s = blockIdx.x; s < ns; s += gridDim.x) {
+            const unsigned long long rec = ws.spill[2 * s];
+            const uint64_t b0 = rec & kBitMask, b1 = ws.spill[2 * s + 1];
+            uint32_t* __restrict__ table = job_ref(one, M_jobs, (int)(rec >> kJobShift)).table;
+            for (uint64_t w = (b0 >> 5) + threadIdx.x; w <= (b1 >> 5); w += blockDim.x) {
+                const uint32_t lo = w == (b0 >> 5) ? (uint32_t)(b0 & 31u) : 0u;
+                const uint32_t hi = w == (b1 >> 5) ? (uint32_t)(b1 & 31u) : 31u;
+                atomicXor(table + w, (0xFFFFFFFFu >> lo) & (0xFFFFFFFFu << (31u - hi)));
+            }
         }
-    }
-}
-
-// Spill list overflow (more than kSpillCap spilled runs): every triangle's
-// spills are recomputed and applied directly.  Exits at once otherwise.
-// Triangles: all of `one`, or of every job of `M` (base = triangle prefix).
-__global__ void __launch_bounds__(256) spill_overflow_kernel(ItemDev one, ItemMap M, MarkerWs ws) {
-    if (ws.misc[1] == 0u) return;
-    const uint64_t total = M.items ? (uint64_t)M.base[M.nitems] : one.info.n_triangles;
-    for (uint64_t gt = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; gt < total;
-         gt += (uint64_t)gridDim.x * blockDim.x) {
-        const int j = M.items ? job_of_triangle(M, (uint32_t)gt) : 0;
-        const ItemDev& it = job_ref(one, M, j);
-        const vox_info& info = it.info;
-        uint32_t* __restrict__ table = it.table;
-        const uint64_t t = M.items ? gt - M.base[j] : gt;
-        const int G = (int)info.gridsize[0];
-        const uint64_t total_bits = (uint64_t)G * G * G;
-        float v[3][3];
-        load_tri(it.tris, t, info, v);
-        SolidTri S;
-        solid_setup(v, info, S);
-        if (!S.valid) continue;
-        const int ylo = (int)S.ymin < 0 ? 0 : (int)S.ymin, zlo = (int)S.zmin < 0 ? 0 : (int)S.zmin;
-        const int yhi = S.ymax >= (float)(G - 1) ? G - 1 : (S.ymax >= 0.0f ? (int)S.ymax : -1);
-        const int zhi = S.zmax >= (float)(G - 1) ? G - 1 : (S.zmax >= 0.0f ? (int)S.zmax : -1);
-        for (int y = ylo; y <= yhi; y++) {
-            const float py = ((float)y + 0.5f) * info.unit[1];
-            for (int z = zlo; z <= zhi; z++) {
-                const float pz = ((float)z + 0.5f) * info.unit[2];
-                if (!column_hit(S, py, pz)) continue;
-                const int64_t xmax = column_xmax(S, info, py, pz);
-                const uint64_t row = (uint64_t)y * G + (uint64_t)z * G * G;
-                if (xmax < (int64_t)G || row + G >= total_bits) continue;
-                const uint64_t b0 = row + G;
-                const uint64_t b1 = (uint64_t)xmax < total_bits - 1 - row ? row + (uint64_t)xmax : total_bits - 1;
-                for (uint64_t w = b0 >> 5; w <= (b1 >> 5); w++) {
-                    const uint32_t lo = w == (b0 >> 5) ? (uint32_t)(b0 & 31u) : 0u;
-                    const uint32_t hi = w == (b1 >> 5) ? (uint32_t)(b1 & 31u) : 31u;
-                    atomicXor(table + w, (0xFFFFFFFFu >> lo) & (0xFFFFFFFFu << (31u - hi)));
+    } else {
+        const uint64_t total = M_tris.items ? (uint64_t)M_tris.base[M_tris.nitems] : one.info.n_triangles;
+        for (uint64_t gt = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; gt < total;
+             gt += (uint64_t)gridDim.x * blockDim.x) {
+            const int j = M_tris.items ? job_of_triangle(M_tris, (uint32_t)gt) : 0;
+            const ItemDev& it = job_ref(one, M_tris, j);
+            const vox_info& info = it.info;
+            uint32_t* __restrict__ table = it.table;
+            const uint64_t t = M_tris.items ? gt - M_tris.base[j] : gt;
+            const int G = (int)info.gridsize[0];
+            const uint64_t total_bits = (uint64_t)G * G * G;
+            float v[3][3];
+            load_tri(it.tris, t, info, v);
+            SolidTri S;
+            solid_setup(v, info, S);
+            if (!S.valid) continue;
+            const int ylo = (int)S.ymin < 0 ? 0 : (int)S.ymin, zlo = (int)S.zmin < 0 ? 0 : (int)S.zmin;
+            const int yhi = S.ymax >= (float)(G - 1) ? G - 1 : (S.ymax >= 0.0f ? (int)S.ymax : -1);
+            const int zhi = S.zmax >= (float)(G - 1) ? G - 1 : (S.zmax >= 0.0f ? (int)S.zmax : -1);
+            for (int y = ylo; y <= yhi; y++) {
+                const float py = ((float)y + 0.5f) * info.unit[1];
+                for (int z = zlo; z <= zhi; z++) {
+                    const float pz = ((float)z + 0.5f) * info.unit[2];
+                    if (!column_hit(S, py, pz)) continue;
+                    const int64_t xmax = column_xmax(S, info, py, pz);
+                    const uint64_t row = (uint64_t)y * G + (uint64_t)z * G * G;
+                    if (xmax < (int64_t)G || row + G >= total_bits) continue;
+                    const uint64_t b0 = row + G;
+                    const uint64_t b1 =
+                        (uint64_t)xmax < total_bits - 1 - row ? row + (uint64_t)xmax : total_bits - 1;
+                    for (uint64_t w = b0 >> 5; w <= (b1 >> 5); w++) {
+                        const uint32_t lo = w == (b0 >> 5) ? (uint32_t)(b0 & 31u) : 0u;
+                        const uint32_t hi = w == (b1 >> 5) ? (uint32_t)(b1 & 31u) : 31u;
+                        atomicXor(table + w, (0xFFFFFFFFu >> lo) & (0xFFFFFFFFu << (31u - hi)));
+                    }
                 }
             }
+        }
+    }
+    __syncthreads();  // this workgroup has read the counters
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&ws.misc[2], 1u) == gridDim.x - 1) {  // every workgroup has read them
+            ws.misc[0] = 0u;
+            ws.misc[1] = 0u;
+            ws.misc[2] = 0u;
+            __threadfence();
         }
     }
 }
@@ -547,6 +556,7 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
         MarkerBuffers& b = g_mws[{dev, stream}];
         if (!b.ws) {
             e = hipMalloc(&b.ws, 256 + 16ull * kSpillCap);
+            if (e == hipSuccess) e = hipMemset(b.ws, 0, 256);  // the counters start at zero
             if (e != hipSuccess) return e;
         }
         if (!in_place && b.scratch_bytes < table_bytes) {
@@ -564,7 +574,6 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
         ws = MarkerWs{(unsigned*)b.ws, (unsigned long long*)((char*)b.ws + 256)};
         if (!in_place) markers = (uint32_t*)b.scratch;
     }
-    marker_init_kernel<<<1, 64, 0, stream>>>(ws.misc);
     e = launch_fill_zero(info, markers, stream);
     if (e != hipSuccess) return e;
     const bool linear = layout == VOX_LAYOUT_LINEAR;
@@ -603,8 +612,7 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
                                                                                  accumulate ? 1 : 0);
     }
     if (linear) {
-        spill_apply_kernel<<<64, 256, 0, stream>>>(fin, none, ws);
-        spill_overflow_kernel<<<256, 256, 0, stream>>>(fin, none, ws);
+        spill_kernel<<<256, 256, 0, stream>>>(fin, none, none, ws);
     }
     return hipGetLastError();
 }
@@ -619,15 +627,15 @@ bool marker_batch_supported(uint32_t G, int layout, bool accumulate) {
 
 // The tables are zero-filled beforehand (the batch's fill class).  m_blocks:
 // block prefix per job, m_rows: suffix unit prefix, m_tris: triangle prefix.
+// mws (marker_ws_bytes()) must be zero when first used; the spill pass
+// leaves it so.
 hipError_t launch_marker_jobs(const ItemMap& m_blocks, const ItemMap& m_rows, const ItemMap& m_tris,
                               uint32_t blocks, uint32_t row_units, void* mws, hipStream_t stream) {
     const MarkerWs ws{(unsigned*)mws, (unsigned long long*)((char*)mws + 256)};
     const ItemDev dummy{};
-    marker_init_kernel<<<1, 64, 0, stream>>>(ws.misc);
     if (blocks) solid_marker_kernel<true><<<blocks, kMBlock, 0, stream>>>(dummy, m_blocks, ws, 1);
     if (row_units) suffix_jobs_kernel<<<row_units, 256, 0, stream>>>(m_rows);
-    spill_apply_kernel<<<64, 256, 0, stream>>>(dummy, m_blocks, ws);
-    spill_overflow_kernel<<<256, 256, 0, stream>>>(dummy, m_tris, ws);
+    spill_kernel<<<256, 256, 0, stream>>>(dummy, m_blocks, m_tris, ws);
     return hipGetLastError();
 }
 
