@@ -30,8 +30,8 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 // Slots tried before a word goes straight to memory: a chunk of scattered
 // triangles has more distinct words than slots, and probing a full table costs
 // more than the atomic it would save.
-constexpr int kProbes = 4;
-constexpr int kBigVox = 64;     // voxel boxes above this (or wider than 32 in x) are queued
+constexpr int kProbes = 2;
+constexpr int kBigVox = 256;    // voxel boxes above this (or wider than 32 in x) are queued
 constexpr int kRecs = 48;       // queued triangles whose records are in LDS at once
 
 struct ChunkShared {
@@ -184,11 +184,12 @@ __device__ __forceinline__ void staged_vertices(const ChunkShared& S, int lt, co
 // comes from the launch's job map; otherwise it is `one` (kernel arguments).
 // LOOP: some job has span > 1 (otherwise the chunk loop is compiled out, which
 // keeps the register count of the one-chunk form down).
-// Five workgroups per CU: the LDS allows five, and the bound keeps the
-// register count at the matching five waves per SIMD (four for the chunk loop,
-// which would otherwise spill).
-template <bool MULTI, bool LOOP>
-__global__ void __launch_bounds__(kChunk, LOOP ? 4 : 5) surface_chunk_kernel(ItemDev one, ItemMap M) {
+// Workgroups per CU: the LDS allows six.  The bound holds the registers at
+// five waves per SIMD, or six with WIDE (a few spilled registers: it pays for
+// the linear layout, whose chunks touch more words and so wait longer on the
+// LDS table, not for the Morton layout), or four for the chunk loop.
+template <bool MULTI, bool LOOP, bool WIDE>
+__global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5)) surface_chunk_kernel(ItemDev one, ItemMap M) {
     __shared__ ChunkShared S;
     __shared__ BlockJob J;
     uint32_t local = blockIdx.x;
@@ -394,13 +395,15 @@ hipError_t launch_fill_jobs(const ItemDev& one, const ItemMap& M, uint32_t units
 hipError_t launch_chunk_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, bool loop, hipStream_t stream) {
     if (units == 0) return hipSuccess;
     if (M.items && loop)
-        surface_chunk_kernel<true, true><<<units, kChunk, 0, stream>>>(one, M);
+        surface_chunk_kernel<true, true, false><<<units, kChunk, 0, stream>>>(one, M);
     else if (M.items)
-        surface_chunk_kernel<true, false><<<units, kChunk, 0, stream>>>(one, M);
+        surface_chunk_kernel<true, false, false><<<units, kChunk, 0, stream>>>(one, M);
     else if (loop)
-        surface_chunk_kernel<false, true><<<units, kChunk, 0, stream>>>(one, M);
+        surface_chunk_kernel<false, true, false><<<units, kChunk, 0, stream>>>(one, M);
+    else if (one.layout == VOX_LAYOUT_LINEAR)
+        surface_chunk_kernel<false, false, true><<<units, kChunk, 0, stream>>>(one, M);
     else
-        surface_chunk_kernel<false, false><<<units, kChunk, 0, stream>>>(one, M);
+        surface_chunk_kernel<false, false, false><<<units, kChunk, 0, stream>>>(one, M);
     return hipGetLastError();
 }
 
