@@ -68,12 +68,12 @@ typedef enum vox_status {
 #define VOX_FLAG_ALGO_CHUNK 0x8u  /* surface only: one workgroup per chunk of
                                     consecutive triangles merges words in an LDS
                                     hash, then one atomicOr per distinct word */
-#define VOX_FLAG_ALGO_MARKER 0x10u /* solid only (G % 32 == 0; Morton: power of two
-                                    up to 4096; otherwise tile): one marker per hit
-                                    column at the run end, gathered in lists per
-                                    block of rows; each block of rows is built in
-                                    LDS as the suffix XOR of its markers along x
-                                    and written once */
+#define VOX_FLAG_ALGO_MARKER 0x10u /* solid only (G % 32 == 0, G <= 16384; Morton:
+                                    power of two up to 4096; otherwise tile): one
+                                    atomicXor marker per hit column at its run end
+                                    in a zeroed bit table (the output itself for
+                                    the linear layout), then every row becomes the
+                                    suffix XOR of its markers along x */
 
 /* ---- setup / diagnostics ---------------------------------------------------- */
 vox_status vox_init(int device);            /* select + probe a device (initCuda) */
@@ -143,7 +143,8 @@ typedef struct vox_batch_item {
     uint64_t* d_offrow;    /* solid only: out-of-row run counter (added to), or NULL */
     int32_t layout;        /* VOX_LAYOUT_LINEAR / VOX_LAYOUT_MORTON */
     int32_t solid;         /* 0 surface, 1 solid */
-    uint32_t flags;        /* VOX_FLAG_ACCUMULATE, VOX_FLAG_ALGO_* */
+    uint32_t flags;        /* VOX_FLAG_ACCUMULATE, VOX_FLAG_ALGO_* (marker: linear,
+                              non-accumulating jobs; others of its kind go to tile) */
     uint32_t _pad;
 } vox_batch_item;
 
