@@ -113,6 +113,11 @@ __device__ __forceinline__ void lane_raster(ChunkShared& S, uint32_t* __restrict
     const float ux = info.unit[0], uy = info.unit[1], uz = info.unit[2];
     const int xa = T.lo[0], xb = T.hi[0];
     const int c0 = xa & ~31, sh = xa - c0;
+    // Morton codes of grids up to 1024 fit 30 bits: the x, y and z spreads are
+    // computed once per lane, row and slice instead of once per emitted mask
+    const bool m32 = layout == VOX_LAYOUT_MORTON && info.gridsize[0] <= 1024u;
+    const uint32_t sc0 = m32 ? spread10((uint32_t)c0) : 0u, sc1 = m32 ? spread10((uint32_t)c0 + 32u) : 0u;
+    const auto put = [&](uint64_t w, uint32_t bits) { word_or(S, table, dense, (uint32_t)w, bits); };
     for (int z = T.lo[2]; z <= T.hi[2]; z++) {
         const float pz = (float)z * uz;
         float azx[3];
@@ -161,6 +166,14 @@ __device__ __forceinline__ void lane_raster(ChunkShared& S, uint32_t* __restrict
             }
             if (!m) continue;
             const uint64_t M = (uint64_t)m << sh;  // bit k = voxel c0 + k
+            if (m32) {
+                const uint32_t syz = (spread10((uint32_t)y) << 1) | (spread10((uint32_t)z) << 2);
+                const uint32_t yz_low =
+                    ((uint32_t)(y & 1) << 1) | ((uint32_t)(z & 1) << 2) | ((uint32_t)((y >> 1) & 1) << 4);
+                if ((uint32_t)M) morton_chunk_words(sc0 | syz, yz_low, (uint32_t)M, put);
+                if (M >> 32) morton_chunk_words(sc1 | syz, yz_low, (uint32_t)(M >> 32), put);
+                continue;
+            }
             emit_mask(S, table, dense, info, layout, c0, y, z, (uint32_t)M);
             if (M >> 32) emit_mask(S, table, dense, info, layout, c0 + 32, y, z, (uint32_t)(M >> 32));
         }

@@ -275,7 +275,9 @@ __device__ __forceinline__ void morton_chunk_words(uint64_t base_code, uint32_t 
         const int g = (__ffs(mask) - 1) >> 2;  // group of 4 x values
         const uint32_t nib = (mask >> (4 * g)) & 0xFu;
         mask &= ~(0xFu << (4 * g));
-        const uint64_t code = base_code | (uint64_t)spread10((uint32_t)(4 * g));
+        // spread10(4 g) for g < 8: x bits 2, 3, 4 land in code bits 6, 9, 12
+        const uint64_t code = base_code | (uint64_t)((((uint32_t)g & 1u) << 6) | (((uint32_t)g & 2u) << 8) |
+                                                     (((uint32_t)g & 4u) << 10));
         uint32_t bits = 0;
         const uint32_t j0 = yz_low;  // j for x offset 0 in the group
         if (nib & 1u) bits |= 1u << (31u - j0);

@@ -129,3 +129,32 @@ def test_batch_marker_spills_per_job(oracle, gpu, use_graph):
             _, ocl, _ = oracle.solid(oracle.info_from_bytes(j.info.as_bytes()), tris)
             assert int(j.offrow.item()) == ocl
     b.close()
+
+
+def test_batch_chunk_work_shapes(oracle, gpu):
+    """One chunk launch holding every work shape: small dense tables with several
+    chunks per workgroup (span), coarse meshes with several workgroups per chunk
+    (parts) and plain one-chunk workgroups."""
+    import torch
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import meshgen
+    fine = np.concatenate([meshgen.torus_slivers(nu=256, nv=128, grid=32, seed=1),
+                           meshgen.triangle_soup(20000, seed=2, mu=-3.0, sigma=1.0)])
+    coarse = meshgen.triangle_soup(700, seed=3, mu=-2.0, sigma=0.8)
+    cases = [(fine, 32, True), (fine, 40, False), (coarse, 128, True), (fine, 64, False), (coarse, 96, False)]
+    jobs, host = [], []
+    for tris, grid, morton in cases:
+        info = V.voxinfo_for_mesh(tris.reshape(-1, 3), grid, tris.shape[0])
+        d = torch.from_numpy(np.ascontiguousarray(tris).reshape(-1)).to(gpu)
+        t = torch.full((V.table_words(grid),), -1, dtype=torch.int32, device=gpu)
+        jobs.append(V.BatchJob(info, d, t, morton=morton, algo="chunk"))
+        host.append(tris)
+    for use_graph in (True, False):
+        b = V.Batch(jobs, use_graph=use_graph)
+        b.launch()
+        _check(oracle, jobs, host)
+        for j in jobs:
+            j.table.fill_(-1)
+        b.launch()
+        _check(oracle, jobs, host)
+        b.close()
