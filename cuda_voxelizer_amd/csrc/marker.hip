@@ -393,21 +393,30 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
                 const uint32_t z_l = (((uint32_t)q >> 3) & 3u) | (z2 << 2);
                 bytes[q] = (tile[(z_l * 32u + y_l) * ts + j] >> bsh) & 0xFFu;
             }
+            // The 256 bits (row q = y0 y1 y2 z0 z1, x offset xo = x0 x1 x2 within the byte, MSB
+            // first) become the 8 words k = z1 | x2 << 1 | y2 << 2, bit 31 - c with
+            // c = x0 | y0 << 1 | z0 << 2 | x1 << 3 | y1 << 4.  Word w = y2 | z0 << 1 | z1 << 2 packs
+            // the bytes of rows (y0, y1) = (0,0) (1,0) (0,1) (1,1) from the top, so bit p of it
+            // has p0 = !x0, p1 = !x1, p2 = !x2, p3 = !y0, p4 = !y1.  Exchanging the low nibbles
+            // of word pair (z0 = 0, 1) with the high ones puts x2 in the word index (z0 = 0
+            // keeps x2 = 0) and !z0 at p2; swapping index bits p1 and p3 (a delta swap) leaves
+            // p = 31 - c.
+            uint32_t W[8];
+#pragma unroll
+            for (int w = 0; w < 8; w++)
+                W[w] = (bytes[4 * w] << 24) | (bytes[4 * w + 1] << 16) | (bytes[4 * w + 2] << 8) | bytes[4 * w + 3];
+#pragma unroll
+            for (int w = 0; w < 8; w++) {
+                if (w & 2) continue;
+                const uint32_t t = (W[w] ^ (W[w | 2] >> 4)) & 0x0F0F0F0Fu;
+                W[w] ^= t;
+                W[w | 2] ^= t << 4;
+            }
             uint32_t out[8];
 #pragma unroll
-            for (int k = 0; k < 8; k++) {  // k = z1 | x2 << 1 | y2 << 2
-                const uint32_t z1 = (uint32_t)k & 1u, x2 = ((uint32_t)k >> 1) & 1u, y2 = ((uint32_t)k >> 2) & 1u;
-                uint32_t word = 0;
-#pragma unroll
-                for (int r = 0; r < 8; r++) {  // r = y0 | y1 << 1 | z0 << 2
-                    const uint32_t y0b = (uint32_t)r & 1u, y1b = ((uint32_t)r >> 1) & 1u, z0b = ((uint32_t)r >> 2) & 1u;
-                    const uint32_t q = (y0b | (y1b << 1) | (y2 << 2)) | ((z0b | (z1 << 1)) << 3);
-                    // byte bit 7 = x offset 0; x2 selects the nibble; nibble bits 3..0 = x0 + 2 x1 = 0..3
-                    const uint32_t nib = x2 ? (bytes[q] & 0xFu) : (bytes[q] >> 4);
-                    const uint32_t c0 = (y0b << 1) | (z0b << 2) | (y1b << 4);
-                    word |= ((nib >> 2) << (30u - c0)) | ((nib & 3u) << (22u - c0));
-                }
-                out[k] = word;
+            for (int w = 0; w < 8; w++) {
+                const uint32_t t = ((W[w] >> 6) ^ W[w]) & 0x00CC00CCu;
+                out[((w >> 2) & 1) | (((w >> 1) & 1) << 1) | ((w & 1) << 2)] = W[w] ^ t ^ (t << 6);
             }
             // word offset inside the block's code range: bits 3 4 5 7 8 from t, then x5.. at 10, 13, ...
             uint64_t wl = ((uint64_t)z2 << 3) | ((uint64_t)x3 << 4) | ((uint64_t)y3 << 5) | ((uint64_t)x4 << 7) |
