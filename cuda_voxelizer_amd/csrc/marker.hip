@@ -317,10 +317,22 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
     const uint32_t rb = blockIdx.x;
     const uint32_t nrows = 32u * rz;
     const uint32_t z0 = (rb / nby) * rz, y0 = (rb % nby) * 32u;
-    // rows (y0.., z) for each z of the block are 32 * rw contiguous words
-    for (uint32_t i = threadIdx.x; i < nrows * rw; i += 256) {
-        const uint32_t zl = i / (32u * rw), k = i % (32u * rw);
-        tile[(i / rw) * ts + i % rw] = markers[((uint64_t)(z0 + zl) * G + y0) * rw + k];
+    // rows (y0.., z) for each z of the block are 32 * rw contiguous words; eight
+    // loads in flight per thread before their LDS stores
+    const uint32_t total = nrows * rw;
+    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += 256u * 8u) {
+        uint32_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint32_t i = i0 + 256u * (uint32_t)u;
+            const uint32_t zl = i / (32u * rw), k = i % (32u * rw);
+            v[u] = i < total ? markers[((uint64_t)(z0 + zl) * G + y0) * rw + k] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint32_t i = i0 + 256u * (uint32_t)u;
+            if (i < total) tile[(i / rw) * ts + i % rw] = v[u];
+        }
     }
     __syncthreads();
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
