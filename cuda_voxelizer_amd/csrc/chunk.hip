@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5)) surface_chu
     do {
         const int count = (int)(end - first < (uint64_t)kChunk ? end - first : (uint64_t)kChunk);
         const float* src = it.tris + first * 9;
-        for (int i = threadIdx.x; i < count * 9; i += kChunk) S.tri[i] = src[i];
+        stage_floats<kChunk>(S.tri, src, count * 9);
         __syncthreads();
         bool big = false;
         SurfTri T;

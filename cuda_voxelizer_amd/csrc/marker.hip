@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(kMBlock) solid_marker_kernel(ItemDev one, Item
     const uint64_t total_bits = (uint64_t)G * G * G;
     if (threadIdx.x == 0) qn = 0;
     const float* src = tris + first * 9;
-    for (int i = threadIdx.x; i < count * 9; i += kMBlock) s_tri[i] = src[i];
+    stage_floats<kMBlock>(s_tri, src, count * 9);
     __syncthreads();
     unsigned long long offrow = 0;
     if ((int)threadIdx.x < count) {
@@ -341,21 +341,31 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
         const uint32_t per = 64 / rw;
         const uint32_t r_in = (uint32_t)lane / rw, j = (uint32_t)lane % rw;
         const bool active_lane = r_in < per;
-        for (uint32_t r0 = (uint32_t)wave * per; r0 < nrows; r0 += 4 * per) {
-            const uint32_t r = r0 + r_in;
-            const bool act = active_lane && r < nrows;
-            const uint32_t w = act ? tile[r * ts + j] : 0u;
-            const unsigned long long par = __ballot(act && (__popc(w) & 1));
-            const uint32_t seg0 = r_in * rw;
-            const unsigned long long above =
-                (active_lane && j + 1 < rw) ? ((par >> (seg0 + j + 1)) & ((1ull << (rw - 1 - j)) - 1ull)) : 0ull;
-            const uint32_t fin = prefix_xor_up(w) ^ ((__popcll(above) & 1) ? 0xFFFFFFFFu : 0u);
-            if (act) {
-                if (linear) {
-                    const uint64_t idx = ((uint64_t)(z0 + r / 32u) * G + y0 + (r & 31u)) * rw + j;
-                    table[idx] = accumulate ? (table[idx] ^ fin) : fin;
-                } else {
-                    tile[r * ts + j] = fin;
+        // four row groups per step: their LDS reads are issued before any result is written
+        for (uint32_t r00 = (uint32_t)wave * per; r00 < nrows; r00 += 16 * per) {
+            uint32_t w[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t r = r00 + 4 * per * (uint32_t)u + r_in;
+                w[u] = active_lane && r < nrows ? tile[r * ts + j] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t r = r00 + 4 * per * (uint32_t)u + r_in;
+                const bool act = active_lane && r < nrows;
+                const unsigned long long par = __ballot(act && (__popc(w[u]) & 1));
+                const uint32_t seg0 = r_in * rw;
+                const unsigned long long above =
+                    (active_lane && j + 1 < rw) ? ((par >> (seg0 + j + 1)) & ((1ull << (rw - 1 - j)) - 1ull))
+                                                : 0ull;
+                const uint32_t fin = prefix_xor_up(w[u]) ^ ((__popcll(above) & 1) ? 0xFFFFFFFFu : 0u);
+                if (act) {
+                    if (linear) {
+                        const uint64_t idx = ((uint64_t)(z0 + r / 32u) * G + y0 + (r & 31u)) * rw + j;
+                        table[idx] = accumulate ? (table[idx] ^ fin) : fin;
+                    } else {
+                        tile[r * ts + j] = fin;
+                    }
                 }
             }
         }

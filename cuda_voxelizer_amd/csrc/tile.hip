@@ -128,7 +128,7 @@ __device__ __forceinline__ void staged_tri(const float* __restrict__ tris, uint6
     const uint64_t cnt = (n - base) < (uint64_t)kBlock ? (n - base) : (uint64_t)kBlock;
     const float* src = tris + base * 9;
     __syncthreads();
-    for (int i = threadIdx.x; i < (int)cnt * 9; i += kBlock) s_tri[i] = src[i];
+    stage_floats<kBlock>(s_tri, src, (int)cnt * 9);
     __syncthreads();
     if (threadIdx.x < cnt) {
 #pragma unroll

@@ -45,6 +45,24 @@ struct SurfTri {
     EdgeTest xy, yz, zx;
 };
 
+// Copies n <= 9 * NT floats (the vertices of up to NT triangles) from memory to
+// LDS with NT threads: each thread issues all its loads before its first LDS
+// store (a plain strided loop waits for every load in turn).
+template <int NT>
+__device__ __forceinline__ void stage_floats(float* dst, const float* __restrict__ src, int n) {
+    float v[9];
+#pragma unroll
+    for (int u = 0; u < 9; u++) {
+        const int i = (int)threadIdx.x + u * NT;
+        v[u] = i < n ? src[i] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < 9; u++) {
+        const int i = (int)threadIdx.x + u * NT;
+        if (i < n) dst[i] = v[u];
+    }
+}
+
 __device__ __forceinline__ void load_tri(const float* __restrict__ tris, uint64_t t, const vox_info& info,
                                          float v[3][3]) {
     const float* p = tris + t * 9;
