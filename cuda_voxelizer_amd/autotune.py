@@ -120,6 +120,19 @@ class Tuner:
         k = self.copies(job)
         return self._time_batch(job, algo, 1), self._time_batch(job, algo, k) / k
 
+    @staticmethod
+    def candidates(job) -> tuple:
+        """Algorithms to time.  direct (one lane per triangle, no work splitting)
+        only for fine meshes (at least G^2/4 triangles): on a coarse mesh its
+        critical path is the largest triangle's whole box on one lane, which
+        copies of the job hide when timed together but which a batch class of
+        its own exposes."""
+        algos = SOLID_ALGOS if job.solid else SURFACE_ALGOS
+        grid = int(job.info.grid)
+        if int(job.info.n_triangles) * 4 < grid * grid:
+            algos = tuple(a for a in algos if a != "direct")
+        return algos
+
     def pick(self, times: dict) -> str:
         """Fastest per copy among the candidates whose single-copy time is
         within latency_slack of the best.  A plain number is both times."""
@@ -136,7 +149,7 @@ class Tuner:
         if hit:
             return hit["algo"]
         timer = self._timer or self._time_gpu
-        times = {a: timer(job, a) for a in (SOLID_ALGOS if job.solid else SURFACE_ALGOS)}
+        times = {a: timer(job, a) for a in self.candidates(job)}
         algo = self.pick(times)
         self.table[key] = {"algo": algo, "seconds": {a: list(t) if isinstance(t, tuple) else t
                                                      for a, t in times.items()}}

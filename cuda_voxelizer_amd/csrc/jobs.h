@@ -18,28 +18,36 @@ struct BlockJob {
     int index;       // job index (0 for a single job)
 };
 
-// Thread 0 resolves, the block reads the result from LDS.  Units past the
-// total (grid rounding) get local = ~0u.
+// The block finds its job with one round of parallel loads (thread t tests
+// whether unit lies in job t's range; a binary search by one thread would
+// wait on ~log2(jobs) dependent loads) and reads the result from LDS.  Units
+// past the total (grid rounding) get local = ~0u.
 __device__ __forceinline__ void resolve_job(const ItemDev& one, const ItemMap& M, uint32_t unit, BlockJob& J) {
-    if (threadIdx.x == 0) {
-        if (M.items == nullptr) {
+    if (M.items == nullptr) {
+        if (threadIdx.x == 0) {
             J.it = one;
             J.local = unit;
             J.index = 0;
-        } else if (unit >= M.base[M.nitems]) {
-            J.it = M.items[0];
-            J.local = ~0u;
-            J.index = 0;
-        } else {
-            int lo = 0, hi = M.nitems - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (M.base[mid] <= unit) lo = mid; else hi = mid - 1;
-            }
-            J.it = M.items[lo];
-            J.local = unit - M.base[lo];
-            J.index = lo;
         }
+    } else {
+        if (threadIdx.x == 0) {
+            J.local = ~0u;  // unless a job claims the unit below
+            J.index = 0;
+        }
+        __syncthreads();
+        for (int j0 = 0; j0 < M.nitems; j0 += (int)blockDim.x) {
+            const int j = j0 + (int)threadIdx.x;
+            if (j < M.nitems) {
+                const uint32_t lo = M.base[j], hi = M.base[j + 1];
+                if (lo <= unit && unit < hi) {  // at most one job (empty ones never match)
+                    J.it = M.items[j];
+                    J.local = unit - lo;
+                    J.index = j;
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && J.local == ~0u) J.it = M.items[0];
     }
     __syncthreads();
 }

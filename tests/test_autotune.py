@@ -69,3 +69,12 @@ def test_pick_rules_out_a_long_critical_path(tmp_path):
     times = {"chunk": (80e-6, 10e-6), "tile": (50e-6, 40e-6), "direct": (300e-6, 5e-6)}
     assert t.pick(times) == "chunk"
     assert t.pick({"tile": 2.0, "direct": 1.0}) == "direct"  # plain numbers: both times
+
+
+def test_direct_is_a_candidate_for_fine_meshes_only(tmp_path):
+    from cuda_voxelizer_amd.autotune import Tuner
+    t = Tuner(str(tmp_path / "t.json"), device_name="fake", timer=lambda job, algo: 1.0)
+    assert "direct" not in t.candidates(_job(256, False, 1000))      # 1000 < 256^2/4
+    assert "direct" in t.candidates(_job(256, False, 20000))
+    assert "direct" not in t.candidates(_job(128, True, 4095))
+    assert set(t.candidates(_job(32, True, 256))) == {"marker", "tile", "direct"}
