@@ -333,10 +333,18 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5)) surface_chu
         __syncthreads();  // S.wq is read above and rewritten by the next chunk
         first += kChunk;
     } while (LOOP && first < end);
-    for (int i = threadIdx.x; i < kHashCap; i += kChunk) {
-        const uint32_t k = S.keys[i], v = S.vals[i];
-        if (k != kEmpty && v != 0u) atomicOr(table + k, v);
+    // every slot read before the first atomic (one LDS wait, not one per slot)
+    static_assert(kHashCap % kChunk == 0, "whole slots per thread");
+    constexpr int kPer = kHashCap / kChunk;
+    uint32_t kk[kPer], vv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+        kk[u] = S.keys[threadIdx.x + u * kChunk];
+        vv[u] = S.vals[threadIdx.x + u * kChunk];
     }
+#pragma unroll
+    for (int u = 0; u < kPer; u++)
+        if (kk[u] != kEmpty && vv[u] != 0u) atomicOr(table + kk[u], vv[u]);
 }
 
 // Zero-fill with 16-byte stores (a kernel, so it can live inside a captured
