@@ -46,11 +46,14 @@ struct ChunkShared {
 };
 static_assert(sizeof(SurfRec) * kRecs <= sizeof(float) * kChunk * 9, "records must fit the staging area");
 
-// dense: the whole table has at most kHashCap words; slot w is preset to key
-// w, so every word is found at its first probe.
-__device__ __forceinline__ void word_or(ChunkShared& S, uint32_t* __restrict__ table, bool dense, uint32_t key,
+// ident: the slot is the key modulo kHashCap.  Used when the whole table has
+// at most kHashCap words (its slots are preset to their keys, so every word is
+// found at its first probe) and for Morton keys (a chunk's words are a compact
+// range of brick codes); linear keys repeat modulo the table size with the z
+// stride, so they are hashed multiplicatively.
+__device__ __forceinline__ void word_or(ChunkShared& S, uint32_t* __restrict__ table, bool ident, uint32_t key,
                                         uint32_t mask) {
-    uint32_t h = dense ? key : (key * 2654435761u) >> (32 - kHashBits);
+    uint32_t h = ident ? (key & (kHashCap - 1)) : (key * 2654435761u) >> (32 - kHashBits);
 #pragma unroll 1
     for (int probe = 0; probe < kProbes; probe++) {
         uint32_t k = S.keys[h];
@@ -69,7 +72,7 @@ __device__ __forceinline__ void word_or(ChunkShared& S, uint32_t* __restrict__ t
 
 // Passing voxels of row (y, z) as a mask (bit k = voxel cx + k, cx a multiple
 // of 32) -> hash, in the table's layout.
-__device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__ table, bool dense,
+__device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__ table, bool ident,
                                           const vox_info& info, int layout, int cx, int y, int z, uint32_t mask) {
     if (!mask) return;
     const uint32_t G = info.gridsize[0];
@@ -79,10 +82,10 @@ __device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__
                                                       (spread10((uint32_t)z) << 2))
                                          : morton3((uint32_t)cx, (uint32_t)y, (uint32_t)z);
         const uint32_t yz_low = ((uint32_t)(y & 1) << 1) | ((uint32_t)(z & 1) << 2) | ((uint32_t)((y >> 1) & 1) << 4);
-        morton_chunk_words(base, yz_low, mask, [&](uint64_t w, uint32_t bits) { word_or(S, table, dense, (uint32_t)w, bits); });
+        morton_chunk_words(base, yz_low, mask, [&](uint64_t w, uint32_t bits) { word_or(S, table, ident, (uint32_t)w, bits); });
     } else if ((G & 31u) == 0) {
         const uint64_t w = ((uint64_t)cx + (uint64_t)y * G + (uint64_t)z * G * G) >> 5;
-        word_or(S, table, dense, (uint32_t)w, __brev(mask));
+        word_or(S, table, ident, (uint32_t)w, __brev(mask));
     } else {
         // linear rows not aligned to words: per voxel
         const uint64_t row = (uint64_t)y * G + (uint64_t)z * G * G;
@@ -93,13 +96,13 @@ __device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__
             const uint64_t idx = row + (uint64_t)(cx + k);
             const uint32_t w = (uint32_t)(idx >> 5);
             if (w != cur) {
-                if (bits) word_or(S, table, dense, cur, bits);
+                if (bits) word_or(S, table, ident, cur, bits);
                 cur = w;
                 bits = 0;
             }
             bits |= bit_mask(idx);
         }
-        if (bits) word_or(S, table, dense, cur, bits);
+        if (bits) word_or(S, table, ident, cur, bits);
     }
 }
 
@@ -108,7 +111,7 @@ __device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__
 // a row then tests the XY edges and the plane only at the x that passed ZX.
 // Every test is the same binary32 expression as in vox_device.h, evaluated
 // for a subset of the voxels: the resulting bits are unchanged.
-__device__ __forceinline__ void lane_raster(ChunkShared& S, uint32_t* __restrict__ table, bool dense,
+__device__ __forceinline__ void lane_raster(ChunkShared& S, uint32_t* __restrict__ table, bool ident,
                                             const SurfTri& T, const vox_info& info, int layout) {
     const float ux = info.unit[0], uy = info.unit[1], uz = info.unit[2];
     const int xa = T.lo[0], xb = T.hi[0];
@@ -117,7 +120,7 @@ __device__ __forceinline__ void lane_raster(ChunkShared& S, uint32_t* __restrict
     // computed once per lane, row and slice instead of once per emitted mask
     const bool m32 = layout == VOX_LAYOUT_MORTON && info.gridsize[0] <= 1024u;
     const uint32_t sc0 = m32 ? spread10((uint32_t)c0) : 0u, sc1 = m32 ? spread10((uint32_t)c0 + 32u) : 0u;
-    const auto put = [&](uint64_t w, uint32_t bits) { word_or(S, table, dense, (uint32_t)w, bits); };
+    const auto put = [&](uint64_t w, uint32_t bits) { word_or(S, table, ident, (uint32_t)w, bits); };
     for (int z = T.lo[2]; z <= T.hi[2]; z++) {
         const float pz = (float)z * uz;
         float azx[3];
@@ -174,8 +177,8 @@ __device__ __forceinline__ void lane_raster(ChunkShared& S, uint32_t* __restrict
                 if (M >> 32) morton_chunk_words(sc1 | syz, yz_low, (uint32_t)(M >> 32), put);
                 continue;
             }
-            emit_mask(S, table, dense, info, layout, c0, y, z, (uint32_t)M);
-            if (M >> 32) emit_mask(S, table, dense, info, layout, c0 + 32, y, z, (uint32_t)(M >> 32));
+            emit_mask(S, table, ident, info, layout, c0, y, z, (uint32_t)M);
+            if (M >> 32) emit_mask(S, table, ident, info, layout, c0 + 32, y, z, (uint32_t)(M >> 32));
         }
     }
 }
@@ -225,6 +228,7 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5)) surface_chu
     const uint32_t G = info.gridsize[0];
     const uint64_t words = ((uint64_t)G * G * G + 31) / 32;
     const bool dense = words <= (uint64_t)kHashCap;
+    const bool ident = dense || layout == VOX_LAYOUT_MORTON;
     for (int i = threadIdx.x; i < kHashCap; i += kChunk) {
         S.keys[i] = dense && (uint64_t)i < words ? (uint32_t)i : kEmpty;
         S.vals[i] = 0u;
@@ -252,7 +256,7 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5)) surface_chu
             float v[3][3];
             staged_vertices(S, threadIdx.x, info, v);
             surface_setup_tests(v, info, T);  // the box is T's already
-            lane_raster(S, table, dense, T, info, layout);  // nx <= 32 here
+            lane_raster(S, table, ident, T, info, layout);  // nx <= 32 here
         }
         __syncthreads();  // S.tri is dead from here on; S.wq is complete
         int myq = -1, nq = 0;  // this lane's queue slot (lane order), queue length
@@ -326,7 +330,7 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5)) surface_chu
                 const int cx = ((bx0 >> 5) + k) << 5;
                 if (cx > bx1) continue;
                 const int x0 = cx > bx0 ? cx : bx0, x1 = (cx + 31) < bx1 ? (cx + 31) : bx1;
-                emit_mask(S, table, dense, info, layout, cx, y, z, row_mask(R.n, R.d1, R.d2, R.xy, R.zx, ux, py, pz, x0, x1, cx));
+                emit_mask(S, table, ident, info, layout, cx, y, z, row_mask(R.n, R.d1, R.d2, R.xy, R.zx, ux, py, pz, x0, x1, cx));
             }
             __syncthreads();  // the records are rewritten by the next round
         }
