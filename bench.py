@@ -37,7 +37,7 @@ CONFIGS = {
     "sh1024s": ("sh8", 1024, False, True, "SH-displaced icosphere L8, 1024^3 linear solid"),
     "sh1024m": ("sh8", 1024, True, True, "SH-displaced icosphere L8, 1024^3 Morton solid"),
 }
-MODES = ("batch256", "serve", "copycal")  # multi-mesh workloads (run_batch / run_serve); PMC calibration copy
+MODES = ("batch256", "serve", "copycal", "rows")  # multi-mesh workloads; PMC calibration copy; the rows around the path
 
 
 def build_mesh(kind: str):
@@ -123,6 +123,9 @@ def main() -> None:
         return
     if args.config == "copycal":
         print(json.dumps(run_copycal(args)), flush=True)
+        return
+    if args.config == "rows":
+        print(json.dumps(run_rows(args)), flush=True)
         return
     kind, grid, morton, solid, desc = CONFIGS[args.config]
     tris = build_mesh(kind)
@@ -223,6 +226,190 @@ def run_copycal(args) -> dict:
     dt = (time.perf_counter() - t0) / k
     return {"metric": "copy GB/s (PMC calibration)", "value": 2 * n / dt / 1e9, "unit": "GB/s", "bytes_read": n,
             "bytes_written": n, "steps": k, "ms_per_step": dt * 1e3}
+
+
+def _event_ms(fn, steps: int, warmup: int, stream) -> float:
+    """Mean device time of fn() between HIP events on `stream`."""
+    import torch
+    for _ in range(max(1, warmup)):
+        fn()
+    torch.cuda.synchronize()
+    s = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    for i in range(steps):
+        s[i].record(stream)
+        fn()
+        e[i].record(stream)
+    torch.cuda.synchronize()
+    return float(np.mean([a.elapsed_time(b) for a, b in zip(s, e)]))
+
+
+def _row(name, ms, alg_bytes, cpu_ms, cpu_sample, same, **extra):
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    out = {"row": name, "ms": ms, "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                                               "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                                               "algorithmic_bytes": alg_bytes},
+           "cpu_baseline": {"ms": cpu_ms, "kind": "port", "sample": cpu_sample}, "identical": bool(same)}
+    out.update(extra)
+    return out
+
+
+def run_rows(args) -> dict:
+    """The rows around the voxelization path, each timed on the GPU against the
+    HBM roofline with its CPU restatement timed beside it and its output checked
+    against that restatement:
+      prep    -- device bbox + AoS packing of the indexed 1024x512-quad torus
+                 (main.cpp:61-80,179), bytes 24 V + 48 F (vertices read for the bbox
+                 and the gather, faces read, triangles written);
+      encode  -- binvox RLE body of the 512^3 linear torus table (util_io.cpp),
+                 bytes G^3/8 read + body written; the C-ABI call returns the
+                 body length, so its time includes one host round trip;
+      convert -- linear -> Morton re-layout of the 1024^3 linear torus table
+                 (voxelize.cuh:20-34), bytes 2 G^3/8, and back;
+      cli     -- the command line end to end (read OBJ, voxelize 256^3, write
+                 binvox) on the reference's test model, wall time."""
+    import ctypes
+    import tempfile
+
+    import torch
+
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import _capi, io, meshgen
+    from oracle import oracle as O
+    lib = _capi.load()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    steps = max(3, min(args.steps, 50))
+    rows = []
+    O.set_threads(min(os.cpu_count() or 1, 16))
+
+    # prep: bbox + packing
+    v, f = meshgen.torus(1024, 512)
+    v = np.ascontiguousarray(v, np.float32)
+    f = np.ascontiguousarray(f, np.int32)
+    dv = torch.from_numpy(v.reshape(-1)).to(dev)
+    df = torch.from_numpy(f.reshape(-1)).to(dev)
+    box = torch.empty(6, dtype=torch.float32, device=dev)
+    tris = torch.empty(f.shape[0] * 9, dtype=torch.float32, device=dev)
+    bad = torch.zeros(1, dtype=torch.int64, device=dev)
+
+    def prep():
+        _capi.check(lib.vox_mesh_bbox(ctypes.c_void_p(dv.data_ptr()), v.shape[0], ctypes.c_void_p(box.data_ptr()), sp))
+        _capi.check(lib.vox_pack_triangles(ctypes.c_void_p(dv.data_ptr()), v.shape[0], ctypes.c_void_p(df.data_ptr()),
+                                           f.shape[0], ctypes.c_void_p(tris.data_ptr()),
+                                           ctypes.c_void_p(bad.data_ptr()), sp))
+    ms = _event_ms(prep, steps, args.warmup, stream)
+    bbox_ms = _event_ms(lambda: _capi.check(lib.vox_mesh_bbox(ctypes.c_void_p(dv.data_ptr()), v.shape[0],
+                                                              ctypes.c_void_p(box.data_ptr()), sp)),
+                        steps, args.warmup, stream)
+    t0 = time.perf_counter()
+    ref_tris = v[f].reshape(-1)
+    ref_box = np.concatenate([v.min(0), v.max(0)])
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    same = np.array_equal(tris.cpu().numpy(), ref_tris) and np.array_equal(box.cpu().numpy(), ref_box)
+    rows.append(_row("prep", ms, 24 * v.shape[0] + 48 * f.shape[0], cpu_ms,
+                     f"numpy gather + min/max, {f.shape[0]} faces, 1 thread", same,
+                     triangles_per_s=f.shape[0] / (ms * 1e-3), bbox_ms=bbox_ms))
+
+    # encode: binvox body of the 512^3 linear torus table
+    G = 512
+    mesh = meshgen.torus_slivers(grid=G)
+    info = V.voxinfo_for_mesh(mesh.reshape(-1, 3), G, mesh.shape[0])
+    d_tris = torch.from_numpy(mesh.reshape(-1)).to(dev)
+    table = V.voxelize_surface(info, d_tris, morton=False)
+    torch.cuda.synchronize()
+    host_table = table.cpu().numpy().view(np.uint32).copy()
+    with io.Encoder(stream) as enc:
+        body = enc.encode(table, G, False, io.FORMATS["binvox"])
+        for _ in range(max(1, args.warmup)):
+            enc.encode(table, G, False, io.FORMATS["binvox"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            enc.encode(table, G, False, io.FORMATS["binvox"])
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+    t0 = time.perf_counter()
+    ref = O.binvox_body(host_table, G)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    rows.append(_row("encode_binvox", ms, G ** 3 // 8 + len(body), cpu_ms,
+                     f"oracle binvox writer (C, 1 thread), {G}^3 table", body == ref, body_bytes=len(body)))
+
+    # convert: 1024^3 linear -> Morton
+    G = 1024
+    mesh = meshgen.torus_slivers(grid=G)
+    info = V.voxinfo_for_mesh(mesh.reshape(-1, 3), G, mesh.shape[0])
+    d_tris = torch.from_numpy(mesh.reshape(-1)).to(dev)
+    lin = V.voxelize_surface(info, d_tris, morton=False)
+    mor = V.voxelize_surface(info, d_tris, morton=True)
+    out_t = torch.empty_like(lin)
+
+    def convert():
+        _capi.check(lib.vox_convert_layout(ctypes.c_void_p(lin.data_ptr()), ctypes.c_void_p(out_t.data_ptr()), G, 1, sp))
+    ms = _event_ms(convert, steps, args.warmup, stream)
+    same = bool(torch.equal(out_t, mor))
+    back_t = torch.empty_like(lin)
+
+    def convert_back():
+        _capi.check(lib.vox_convert_layout(ctypes.c_void_p(mor.data_ptr()), ctypes.c_void_p(back_t.data_ptr()), G, 0,
+                                           sp))
+    back_ms = _event_ms(convert_back, steps, args.warmup, stream)
+    same = same and bool(torch.equal(back_t, lin))
+    # CPU: the same permutation in numpy on a 256^3 sample (its own torus table)
+    g = 256
+    small = meshgen.torus_slivers(grid=g)
+    sinfo = V.voxinfo_for_mesh(small.reshape(-1, 3), g, small.shape[0])
+    s_lin = V.voxelize_surface(sinfo, torch.from_numpy(small.reshape(-1)).to(dev), morton=False).cpu().numpy()
+    s_mor = V.voxelize_surface(sinfo, torch.from_numpy(small.reshape(-1)).to(dev), morton=True).cpu().numpy()
+    t0 = time.perf_counter()
+    bits = np.unpackbits(s_lin.view(np.uint32).byteswap().view(np.uint8))  # voxel idx order, MSB first
+    c = np.arange(g, dtype=np.uint64)
+    sp3 = np.zeros(g, np.uint64)
+    for k in range(10):
+        sp3 |= ((c >> np.uint64(k)) & np.uint64(1)) << np.uint64(3 * k)
+    code = (sp3[None, None, :] | (sp3[None, :, None] << np.uint64(1)) | (sp3[:, None, None] << np.uint64(2))).reshape(-1)
+    mbits = np.zeros_like(bits)
+    mbits[code] = bits
+    c_mor = np.packbits(mbits).view(np.uint32).byteswap()
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    same = same and np.array_equal(c_mor, s_mor.view(np.uint32))
+    rows.append(_row("convert_linear_to_morton", ms, 2 * G ** 3 // 8, cpu_ms * (G // g) ** 3,
+                     f"numpy permutation of a {g}^3 table, scaled x{(G // g) ** 3} to {G}^3", same,
+                     morton_to_linear_ms=back_ms,
+                     morton_to_linear_frac=2 * G ** 3 / 8 / (back_ms * 1e-3) / 1e9 / HBM_PEAK_GBS))
+
+    # cli: end to end on the reference's test model
+    from cuda_voxelizer_amd import cli
+    with tempfile.TemporaryDirectory() as td:
+        g = np.load(os.path.join(ROOT, "tests", "golden", "bunny.npz"))
+        obj = os.path.join(td, "bunny.obj")
+        with open(obj, "w") as fh:
+            for p in g["vertices"]:
+                fh.write(f"v {p[0]:.9g} {p[1]:.9g} {p[2]:.9g}\n")
+            for t in g["faces"]:
+                fh.write(f"f {t[0] + 1} {t[1] + 1} {t[2] + 1}\n")
+        import contextlib
+        import io as pyio
+        with contextlib.redirect_stdout(pyio.StringIO()):
+            cli.main(["-f", obj, "-s", "256", "-o", "binvox"])  # warm
+            t0 = time.perf_counter()
+            cli.main(["-f", obj, "-s", "256", "-o", "binvox"])
+            ms = (time.perf_counter() - t0) * 1e3
+        with open(obj + "_256.binvox", "rb") as fh:
+            got = fh.read()
+        vv, ff = io.load_mesh(obj)
+        oinfo = O.info_for_mesh(vv, 256, ff.shape[0])
+        t0 = time.perf_counter()
+        otab, _ = O.surface(oinfo, vv[ff].astype(np.float32))
+        oref = O.binvox_body(otab, 256)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        same = got.endswith(oref)
+    rows.append({"row": "cli_binvox_256", "ms": ms, "cpu_baseline": {"ms": cpu_ms, "kind": "port",
+                 "sample": "oracle voxelization (16 threads) + binvox writer of the same model"},
+                 "identical": bool(same), "faces": int(ff.shape[0])})
+    return {"metric": "rows around the path: device ms, HBM roofline, CPU restatement", "value": None,
+            "unit": "ms", "n_gpus": 1, "steps": steps, "warmup": args.warmup, "higher_is_better": False,
+            "data": "synthetic torus + the reference's bunny", "rows": rows}
 
 
 def _batch_jobs(meshes, dev):

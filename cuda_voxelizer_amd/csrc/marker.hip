@@ -308,6 +308,9 @@ __global__ void __launch_bounds__(256) suffix_jobs_kernel(ItemMap M) {
 // contiguous rows.  Morton output: a word is a 4x4x2 brick (code bits 0..4 =
 // x0 y0 z0 x1 y1); a thread takes one linear word (32 x) of the 8 rows of a
 // brick row and emits the 8 Morton words of its 8 groups of 4 x.
+// SUFFIX = false: the rows are copied as they are (linear -> Morton layout
+// conversion of a finished table).
+template <bool SUFFIX>
 __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __restrict__ markers,
                                                            uint32_t* __restrict__ table, uint32_t G, uint32_t rz,
                                                            int layout, int accumulate) {
@@ -337,7 +340,9 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
     __syncthreads();
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
     const bool linear = layout == VOX_LAYOUT_LINEAR;
-    if (rw <= 64) {
+    if (!SUFFIX) {
+        // rows stay as loaded
+    } else if (rw <= 64) {
         const uint32_t per = 64 / rw;
         const uint32_t r_in = (uint32_t)lane / rw, j = (uint32_t)lane % rw;
         const bool active_lane = r_in < per;
@@ -556,6 +561,74 @@ __global__ void __launch_bounds__(256) spill_kernel(ItemDev one, ItemMap M_jobs,
     }
 }
 
+// Morton -> linear layout, the inverse of the brick assembly above: one
+// workgroup per block of 32 rows in y x 8 rows in z (G power of two, 32..1024);
+// a thread reads its 8 consecutive Morton words (two 16-byte loads), undoes the
+// index swap and the nibble exchange (both are involutions), scatters the 32
+// bytes into the LDS rows, and the block then stores its rows coalesced.
+__global__ void __launch_bounds__(256) bricks_to_rows_kernel(const uint32_t* __restrict__ src,
+                                                             uint32_t* __restrict__ dst, uint32_t G) {
+    extern __shared__ uint32_t tile[];  // [8 * 32 rows][rw + 1]
+    const uint32_t rw = G >> 5, nby = G >> 5, ts = rw + 1;
+    const uint32_t rb = blockIdx.x;
+    const uint32_t z0 = (rb / nby) * 8u, y0 = (rb % nby) * 32u;
+    const uint64_t base_w = morton3(0u, y0, z0) >> 5;
+    uint8_t* tb = (uint8_t*)tile;
+    for (uint32_t t = threadIdx.x; t < G; t += 256) {
+        const uint32_t z2 = t & 1u, x3 = (t >> 1) & 1u, y3 = (t >> 2) & 1u, x4 = (t >> 3) & 1u,
+                       y4 = (t >> 4) & 1u, xh = t >> 5;
+        uint64_t wl = ((uint64_t)z2 << 3) | ((uint64_t)x3 << 4) | ((uint64_t)y3 << 5) | ((uint64_t)x4 << 7) |
+                      ((uint64_t)y4 << 8);
+        for (uint32_t b = 0; (xh >> b) != 0u; b++) wl |= (uint64_t)((xh >> b) & 1u) << (10u + 3u * b);
+        const uint4* s4 = (const uint4*)(src + base_w + wl);
+        const uint4 lo = s4[0], hi = s4[1];
+        const uint32_t out[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+        uint32_t W[8];
+#pragma unroll
+        for (int w = 0; w < 8; w++) {
+            const uint32_t v = out[((w >> 2) & 1) | (((w >> 1) & 1) << 1) | ((w & 1) << 2)];
+            const uint32_t d = ((v >> 6) ^ v) & 0x00CC00CCu;
+            W[w] = v ^ d ^ (d << 6);
+        }
+#pragma unroll
+        for (int w = 0; w < 8; w++) {
+            if (w & 2) continue;
+            const uint32_t d = (W[w] ^ (W[w | 2] >> 4)) & 0x0F0F0F0Fu;
+            W[w] ^= d;
+            W[w | 2] ^= d << 4;
+        }
+        const uint32_t xbyte = x3 | (x4 << 1) | (xh << 2);  // x >> 3
+        const uint32_t j = xbyte >> 2, byte_in_word = 3u - (xbyte & 3u);  // MSB-first byte -> little-endian slot
+#pragma unroll
+        for (int q = 0; q < 32; q++) {
+            const uint32_t y_l = ((uint32_t)q & 7u) | (y3 << 3) | (y4 << 4);
+            const uint32_t z_l = (((uint32_t)q >> 3) & 3u) | (z2 << 2);
+            const uint32_t byte = (W[q >> 2] >> (24u - 8u * ((uint32_t)q & 3u))) & 0xFFu;
+            tb[4u * ((z_l * 32u + y_l) * ts + j) + byte_in_word] = (uint8_t)byte;
+        }
+    }
+    __syncthreads();
+    const uint32_t total = 256u * rw;
+    for (uint32_t i = threadIdx.x; i < total; i += 256) {
+        const uint32_t zl = i / (32u * rw), k = i % (32u * rw);
+        dst[((uint64_t)(z0 + zl) * G + y0) * rw + k] = tile[(i / rw) * ts + i % rw];
+    }
+}
+
+bool layout_convert_fast(uint32_t G) { return G >= 32u && G <= 1024u && (G & (G - 1)) == 0; }
+
+// Whole-table layout conversion for a power-of-two G in 32..1024; 16-byte
+// aligned tables.
+hipError_t launch_layout_convert(const uint32_t* src, uint32_t* dst, uint32_t G, bool to_morton, hipStream_t stream) {
+    const uint32_t nblocks = (G / 32u) * (G / 8u);
+    const size_t lds = 4ull * 256u * (G / 32u + 1u);
+    if (to_morton)
+        marker_build_kernel<false><<<nblocks, 256, lds, stream>>>(src, dst, G, 8u, VOX_LAYOUT_MORTON, 0);
+    else
+        bricks_to_rows_kernel<<<nblocks, 256, lds, stream>>>(src, dst, G);
+    return hipGetLastError();
+}
+
 // ---- host ----------------------------------------------------------------------
 struct MarkerBuffers {
     void* ws = nullptr;       // misc + spill records
@@ -639,7 +712,7 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
     } else {
         const uint32_t rz = linear ? 1u : (G <= 1024u ? 8u : 2u);
         const uint32_t nblocks = (G / 32u) * (G / rz);
-        marker_build_kernel<<<nblocks, 256, 4ull * 32u * rz * (G / 32u + 1u), stream>>>(markers, table, G, rz, layout,
+        marker_build_kernel<true><<<nblocks, 256, 4ull * 32u * rz * (G / 32u + 1u), stream>>>(markers, table, G, rz, layout,
                                                                                  accumulate ? 1 : 0);
     }
     if (linear) {

@@ -18,6 +18,7 @@
 #include "../../include/cuda_voxelizer_amd/vox_capi.h"
 #include "vox_device.h"
 #include "vox_host.h"
+#include "vox_kernels.h"
 
 namespace voxamd {
 
@@ -35,8 +36,16 @@ __global__ void __launch_bounds__(256) bbox_init_kernel(uint32_t* keys) {
     if (threadIdx.x >= 3 && threadIdx.x < 6) keys[threadIdx.x] = 0u;  // max
 }
 
+// Grid-stride min/max keys, reduced over the wave (shuffles) and the block
+// (LDS) before one atomicMin/atomicMax per key and block: at most kBboxBlocks
+// atomics per address (one per wave cost ~8K same-address atomics for a
+// 512K-vertex mesh, serialised at the memory side).
+constexpr unsigned kBboxBlocks = 512;
+
 __global__ void __launch_bounds__(256) bbox_kernel(const float* __restrict__ v, uint64_t n, uint32_t* keys) {
+    __shared__ uint32_t part[4][6];
     uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+#pragma unroll 4
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -53,12 +62,21 @@ __global__ void __launch_bounds__(256) bbox_kernel(const float* __restrict__ v, 
             mx[k] = b > mx[k] ? b : mx[k];
         }
     }
+    const int wave = threadIdx.x >> 6;
     if (__lane_id() == 0) {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            atomicMin(keys + k, mn[k]);
-            atomicMax(keys + 3 + k, mx[k]);
+            part[wave][k] = mn[k];
+            part[wave][3 + k] = mx[k];
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        uint32_t r = part[0][k];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = k < 3 ? (part[w][k] < r ? part[w][k] : r)
+                                                                  : (part[w][k] > r ? part[w][k] : r);
+        if (k < 3) atomicMin(keys + k, r); else atomicMax(keys + k, r);
     }
 }
 
@@ -83,7 +101,8 @@ __global__ void __launch_bounds__(256) pack_kernel(const float* __restrict__ v, 
     if (bad && nbad) atomicAdd(bad, nbad);
 }
 
-// Linear <-> Morton: one thread per destination word gathers its 32 voxels.
+// Linear <-> Morton for the grids the block form does not take (G < 32,
+// G > 1024, unaligned tables): one thread per destination word gathers its 32 voxels.
 __global__ void __launch_bounds__(256) convert_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
                                                       uint32_t G, int to_morton) {
     const uint64_t words = ((uint64_t)G * G * G) >> 5;
@@ -147,7 +166,8 @@ vox_status vox_mesh_bbox(const float* d_vertices, uint64_t n_vertices, float* d_
     // the 6 keys live in the output buffer itself until the finish kernel converts them
     uint32_t* keys = (uint32_t*)d_out6;
     bbox_init_kernel<<<1, 64, 0, st>>>(keys);
-    bbox_kernel<<<grid_for(n_vertices), 256, 0, st>>>(d_vertices, n_vertices, keys);
+    const unsigned nb = grid_for(n_vertices) < kBboxBlocks ? grid_for(n_vertices) : kBboxBlocks;
+    bbox_kernel<<<nb, 256, 0, st>>>(d_vertices, n_vertices, keys);
     bbox_finish_kernel<<<1, 64, 0, st>>>(keys, d_out6);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? VOX_OK : hip_fail(e, "bbox kernels");
@@ -194,8 +214,14 @@ vox_status vox_convert_layout(const uint32_t* d_src, uint32_t* d_dst, uint32_t g
     if (grid < 4 || (grid & (grid - 1)) || grid > kMaxGrid)
         return fail(VOX_ERR_UNSUPPORTED, "layout conversion needs a power-of-two grid >= 4");
     const uint64_t words = ((uint64_t)grid * grid * grid) >> 5;
-    convert_kernel<<<grid_for(words), 256, 0, (hipStream_t)stream>>>(d_src, d_dst, grid, to_morton ? 1 : 0);
-    hipError_t e = hipGetLastError();
+    hipError_t e;
+    if (layout_convert_fast(grid) && ((((uintptr_t)d_src) | ((uintptr_t)d_dst)) & 15u) == 0) {
+        // blocks of 32 x 8 rows through LDS, 8 Morton words per thread (marker.hip)
+        e = launch_layout_convert(d_src, d_dst, grid, to_morton != 0, (hipStream_t)stream);
+    } else {
+        convert_kernel<<<grid_for(words), 256, 0, (hipStream_t)stream>>>(d_src, d_dst, grid, to_morton ? 1 : 0);
+        e = hipGetLastError();
+    }
     return e == hipSuccess ? VOX_OK : hip_fail(e, "convert kernel");
 }
 

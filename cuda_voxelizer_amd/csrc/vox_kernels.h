@@ -27,6 +27,8 @@ hipError_t launch_tiled(const vox_info& info, const float* tris, uint32_t* table
 
 bool marker_supported(uint32_t G, int layout, bool accumulate);
 bool marker_batch_supported(uint32_t G, int layout, bool accumulate);
+bool layout_convert_fast(uint32_t G);
+hipError_t launch_layout_convert(const uint32_t* src, uint32_t* dst, uint32_t G, bool to_morton, hipStream_t stream);
 uint64_t marker_ws_bytes();
 uint32_t marker_block_units(uint64_t n_triangles);
 uint32_t marker_row_units(uint32_t G);

@@ -29,7 +29,7 @@ def test_pack_rejects_bad_indices(gpu):
         V.pack_triangles(vtx, faces)
 
 
-@pytest.mark.parametrize("grid", [4, 32, 64])
+@pytest.mark.parametrize("grid", [4, 16, 32, 64, 128, 256, 1024, 2048])
 def test_layout_conversion_round_trip(oracle, gpu, grid):
     import torch
     import cuda_voxelizer_amd as V
@@ -43,3 +43,32 @@ def test_layout_conversion_round_trip(oracle, gpu, grid):
     assert (V.convert_layout(mor, grid, False) == lin).all()
     n = V.count_voxels(lin)
     assert n == V.count_voxels(mor) == int(np.unpackbits(lin.cpu().numpy().view(np.uint8)).sum())
+
+
+def _morton_codes(g):
+    c = np.arange(g, dtype=np.uint64)
+    sp = np.zeros(g, np.uint64)
+    for k in range(12):
+        sp |= ((c >> np.uint64(k)) & np.uint64(1)) << np.uint64(3 * k)
+    # [z, y, x] -> code of (x, y, z): x bits at 3k, y at 3k+1, z at 3k+2
+    return (sp[None, None, :] | (sp[None, :, None] << np.uint64(1)) | (sp[:, None, None] << np.uint64(2))).reshape(-1)
+
+
+@pytest.mark.parametrize("grid", [8, 32, 64, 256])
+def test_layout_conversion_random_bits(gpu, grid):
+    """Random tables both ways against a numpy restatement of the two layouts
+    (linear x + y G + z G^2, Morton bit interleave, MSB-first words)."""
+    import torch
+    import cuda_voxelizer_amd as V
+    rng = np.random.default_rng(grid)
+    words = grid ** 3 // 32
+    lin = rng.integers(0, 2 ** 32, size=words, dtype=np.uint64).astype(np.uint32)
+    bits = np.unpackbits(lin.byteswap().view(np.uint8))  # voxel order, MSB first
+    code = _morton_codes(grid)
+    mbits = np.zeros_like(bits)
+    mbits[code] = bits
+    mor = np.packbits(mbits).view(np.uint32).byteswap()
+    d_lin = torch.from_numpy(lin.view(np.int32)).to(gpu)
+    d_mor = torch.from_numpy(mor.view(np.int32)).to(gpu)
+    assert np.array_equal(V.convert_layout(d_lin, grid, True).cpu().numpy().view(np.uint32), mor)
+    assert np.array_equal(V.convert_layout(d_mor, grid, False).cpu().numpy().view(np.uint32), lin)
