@@ -112,6 +112,8 @@ def test_batch_marker_spills_per_job(oracle, gpu, use_graph):
     v, f = meshgen.icosphere_sh(level=4, seed=3)  # a closed mesh without spills in the same class
     sh = meshgen.to_aos(v, f)
     cases.insert(2, (V.voxinfo_for_mesh(sh.reshape(-1, 3), 64, sh.shape[0]), sh))
+    empty = np.zeros((0, 3, 3), np.float32)  # no triangles: the table is still defined (all zero)
+    cases.insert(1, (V.VoxInfo(V.AABox(np.zeros(3, np.float32), np.ones(3, np.float32)), 32, 0), empty))
     for info, tris in cases:
         d = torch.from_numpy(np.ascontiguousarray(tris).reshape(-1)).to(gpu)
         t = torch.full((V.table_words(info.grid),), -1, dtype=torch.int32, device=gpu)
