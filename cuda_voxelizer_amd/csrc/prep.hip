@@ -40,18 +40,29 @@ __global__ void __launch_bounds__(256) bbox_init_kernel(uint32_t* keys) {
 // (LDS) before one atomicMin/atomicMax per key and block: at most kBboxBlocks
 // atomics per address (one per wave cost ~8K same-address atomics for a
 // 512K-vertex mesh, serialised at the memory side).
-constexpr unsigned kBboxBlocks = 512;
+constexpr unsigned kBboxBlocks = 64;
 
 __global__ void __launch_bounds__(256) bbox_kernel(const float* __restrict__ v, uint64_t n, uint32_t* keys) {
     __shared__ uint32_t part[4][6];
     uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
-#pragma unroll 4
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    // eight vertices per step, their 24 loads issued before any is used
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i0 < n; i0 += 8 * stride) {
+        float f[8][3];
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const uint32_t key = float_key(v[3 * i + k]);
-            mn[k] = key < mn[k] ? key : mn[k];
-            mx[k] = key > mx[k] ? key : mx[k];
+        for (int u = 0; u < 8; u++) {
+            const uint64_t i = i0 + (uint64_t)u * stride;
+#pragma unroll
+            for (int k = 0; k < 3; k++) f[u][k] = i < n ? v[3 * i + k] : v[3 * i0 + k];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const uint32_t key = float_key(f[u][k]);
+                mn[k] = key < mn[k] ? key : mn[k];
+                mx[k] = key > mx[k] ? key : mx[k];
+            }
         }
     }
 #pragma unroll
@@ -84,19 +95,52 @@ __global__ void __launch_bounds__(64) bbox_finish_kernel(const uint32_t* keys, f
     if (threadIdx.x < 6) out[threadIdx.x] = key_float(keys[threadIdx.x]);
 }
 
+// One workgroup per 256 faces: indices and triangles pass through LDS so the
+// global loads and stores are coalesced (a face is 12 bytes in, 36 bytes out);
+// each lane gathers its face's three vertices.
 __global__ void __launch_bounds__(256) pack_kernel(const float* __restrict__ v, uint64_t nv,
                                                    const int32_t* __restrict__ faces, uint64_t nf,
                                                    float* __restrict__ out, unsigned long long* bad) {
+    __shared__ int32_t s_idx[256 * 3];
+    __shared__ float s_out[256 * 9];
     unsigned long long nbad = 0;
-    for (uint64_t f = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; f < nf; f += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint64_t base = blockIdx.x * 256ull; base < nf; base += (uint64_t)gridDim.x * 256ull) {
+        const int cnt = (int)(nf - base < 256ull ? nf - base : 256ull);
+        const int32_t* fsrc = faces + 3 * base;
+        int32_t fi[3];
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const int32_t idx = faces[3 * f + c];
-            const bool ok = idx >= 0 && (uint64_t)idx < nv;
-            if (!ok) nbad++;
-#pragma unroll
-            for (int k = 0; k < 3; k++) out[9 * f + 3 * c + k] = ok ? v[3 * (uint64_t)idx + k] : 0.0f;
+        for (int u = 0; u < 3; u++) {
+            const int i = (int)threadIdx.x + 256 * u;
+            fi[u] = i < 3 * cnt ? fsrc[i] : 0;
         }
+#pragma unroll
+        for (int u = 0; u < 3; u++) {
+            const int i = (int)threadIdx.x + 256 * u;
+            if (i < 3 * cnt) s_idx[i] = fi[u];
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < cnt) {
+            float t[9];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const int32_t idx = s_idx[3 * threadIdx.x + c];
+                const bool ok = idx >= 0 && (uint64_t)idx < nv;
+                if (!ok) nbad++;
+                const uint64_t src = ok ? 3 * (uint64_t)idx : 0;
+#pragma unroll
+                for (int k = 0; k < 3; k++) t[3 * c + k] = ok ? v[src + k] : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < 9; k++) s_out[9 * threadIdx.x + k] = t[k];
+        }
+        __syncthreads();
+        float* dst = out + 9 * base;
+#pragma unroll
+        for (int u = 0; u < 9; u++) {
+            const int i = (int)threadIdx.x + 256 * u;
+            if (i < 9 * cnt) dst[i] = s_out[i];
+        }
+        __syncthreads();  // s_idx / s_out are reused by the next group
     }
     if (bad && nbad) atomicAdd(bad, nbad);
 }

@@ -72,3 +72,20 @@ def test_layout_conversion_random_bits(gpu, grid):
     d_mor = torch.from_numpy(mor.view(np.int32)).to(gpu)
     assert np.array_equal(V.convert_layout(d_lin, grid, True).cpu().numpy().view(np.uint32), mor)
     assert np.array_equal(V.convert_layout(d_mor, grid, False).cpu().numpy().view(np.uint32), lin)
+
+
+@pytest.mark.parametrize("nv,nf", [(3, 1), (777, 1000), (70001, 300001)])
+def test_bbox_and_pack_odd_sizes(gpu, nv, nf):
+    """Partial groups of faces and vertices: device bbox and packing equal numpy."""
+    import torch
+    import cuda_voxelizer_amd as V
+    rng = np.random.default_rng(nv)
+    v = rng.normal(size=(nv, 3)).astype(np.float32) * np.float32(3.0) - np.float32(1.0)
+    f = rng.integers(0, nv, size=(nf, 3)).astype(np.int32)
+    dv = torch.from_numpy(v.reshape(-1)).to(gpu)
+    df = torch.from_numpy(f.reshape(-1)).to(gpu)
+    box = V.mesh_bbox_device(dv)
+    np.testing.assert_array_equal(box.min, v.min(0))
+    np.testing.assert_array_equal(box.max, v.max(0))
+    tris = V.pack_triangles(dv, df)
+    np.testing.assert_array_equal(tris.cpu().numpy(), v[f].reshape(-1))
