@@ -17,9 +17,13 @@
 // binvox runs: the reference writes (value, count) byte pairs, a new pair at
 // every value change and after 255 equal voxels (util_io.cpp:228-262).  So a
 // pair starts at every value change s and at s + 255k inside the run that
-// starts at s.  The run start in force at each word's first voxel is an
-// exclusive max-scan of the last change position per word; each pair's
-// count is the distance to the next pair start (the last one: to G^3).
+// starts at s.  binvox works on tiles of 2048 words (8 per thread): the
+// last change position per tile, an exclusive max-scan of those (one
+// workgroup) gives the run start in force at each tile, a block-level max-scan
+// inside the tile the one at each thread's words; pairs are counted per tile,
+// the counts scanned (one workgroup), and each tile emits its pairs from its
+// offset.  Each pair's count is the distance to the next pair start (the last
+// one: to G^3).
 //
 // Text formats follow std::ostream's default formatting (precision 6, %g):
 // x + 0.5 for x < 10^5 prints as the integer digits followed by ".5".
@@ -132,26 +136,6 @@ __global__ void __launch_bounds__(256) permute_gather_kernel(const uint32_t* __r
 // ---------------------------------------------------------------------------
 // 2 + 4: binvox run-length pairs
 // ---------------------------------------------------------------------------
-// Change mask of word w: bit 31-i set iff voxel 32w+i differs from the voxel
-// before it (voxel 0 always starts a run).  Positions >= n are cut off.
-__device__ __forceinline__ uint32_t change_mask(const uint32_t* P, uint64_t w, uint64_t n) {
-    const uint32_t bits = P[w];
-    const uint32_t prev = w ? (P[w - 1] & 1u) : ((~bits >> 31) & 1u);
-    uint32_t B = bits ^ ((bits >> 1) | (prev << 31));
-    const uint64_t left = n - w * 32;
-    if (left < 32) B &= ~(0xFFFFFFFFu >> left);  // keep the first `left` positions
-    return B;
-}
-
-__global__ void __launch_bounds__(256) rle_last_change_kernel(const uint32_t* __restrict__ P, uint64_t words,
-                                                              uint64_t n, int64_t* __restrict__ last) {
-    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < words;
-         w += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t B = change_mask(P, w, n);
-        last[w] = B ? (int64_t)(w * 32 + 31 - (uint64_t)__ffs(B) + 1) : -1;
-    }
-}
-
 // Pair starts s + 255k (k >= 0) inside [a, b], a >= s.
 __device__ __forceinline__ uint32_t pairs_in(int64_t s, int64_t a, int64_t b) {
     if (b < a) return 0u;
@@ -159,45 +143,140 @@ __device__ __forceinline__ uint32_t pairs_in(int64_t s, int64_t a, int64_t b) {
     return hi >= lo ? (uint32_t)(hi - lo + 1) : 0u;
 }
 
-// EMIT = false: pairs starting in each word -> cnt.  EMIT = true: the pairs'
-// start positions and value bytes at their offsets.
+// The pairs that start in one word (positions p0 .. end-1, change mask B, run
+// start s in force at p0, updated to the word's last change).  EMIT: their
+// start positions and value bytes from offset o.  Returns their number.
 template <bool EMIT>
-__global__ void __launch_bounds__(256) rle_pairs_kernel(const uint32_t* __restrict__ P, uint64_t words, uint64_t n,
-                                                        const uint64_t* __restrict__ run_in,
-                                                        uint32_t* __restrict__ cnt,
-                                                        const uint64_t* __restrict__ off,
-                                                        uint64_t* __restrict__ starts, uint8_t* __restrict__ out) {
-    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < words;
-         w += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t bits = P[w];
-        uint32_t B = change_mask(P, w, n);
-        const int64_t p0 = (int64_t)(w * 32);
-        const int64_t end = (int64_t)((n - w * 32) < 32 ? n : w * 32 + 32);  // one past the last position
-        int64_t s = (int64_t)run_in[w];
-        int64_t a = p0;
-        uint32_t count = 0;
-        uint64_t o = EMIT ? off[w] : 0;
-        for (;;) {
-            const int i = B ? __clz(B) : 32;  // next change offset in the word
-            const int64_t b = (i < 32 ? p0 + i : end) - 1;
-            if (b >= a) {
-                if (EMIT) {
-                    for (int64_t q = s + ((a - s + 254) / 255) * 255; q <= b; q += 255) {
-                        starts[o] = (uint64_t)q;
-                        out[2 * o] = (uint8_t)((bits >> (31 - (int)(q - p0))) & 1u);
-                        o++;
-                    }
-                } else {
-                    count += pairs_in(s, a, b);
+__device__ __forceinline__ uint32_t word_pairs(uint32_t bits, uint32_t B, int64_t p0, int64_t end, int64_t& s,
+                                               uint64_t o, uint64_t* __restrict__ starts, uint8_t* __restrict__ out) {
+    int64_t a = p0;
+    uint32_t count = 0;
+    for (;;) {
+        const int i = B ? __clz(B) : 32;  // next change offset in the word
+        const int64_t b = (i < 32 ? p0 + i : end) - 1;
+        if (b >= a) {
+            if (EMIT) {
+                for (int64_t q = s + ((a - s + 254) / 255) * 255; q <= b; q += 255) {
+                    starts[o] = (uint64_t)q;
+                    out[2 * o] = (uint8_t)((bits >> (31 - (int)(q - p0))) & 1u);
+                    o++;
+                    count++;
                 }
+            } else {
+                count += pairs_in(s, a, b);
             }
-            if (i >= 32) break;
-            B &= ~(0x80000000u >> i);
-            s = p0 + i;
-            a = s;
         }
-        if (!EMIT) cnt[w] = count;
+        if (i >= 32) break;
+        B &= ~(0x80000000u >> i);
+        s = p0 + i;
+        a = s;
     }
+    return count;
+}
+
+// Tiles of kRleTile words, kRlePer consecutive words per thread: a thread's
+// words, their change masks and its last change position.
+constexpr int kRlePer = 8;
+constexpr uint64_t kRleTile = 256ull * kRlePer;
+
+// Change mask B of word w: bit 31-i set iff voxel 32w+i differs from the
+// voxel before it (voxel 0 always starts a run); positions >= n cut off.
+struct RleWords {
+    uint32_t bits[kRlePer], B[kRlePer];
+    int64_t last;  // last change position in the thread's words, or -1
+};
+
+__device__ __forceinline__ void rle_words(const uint32_t* __restrict__ P, uint64_t words, uint64_t n, uint64_t w0,
+                                          RleWords& R) {
+    const uint32_t before = (w0 > 0 && w0 - 1 < words) ? P[w0 - 1] : 0u;
+#pragma unroll
+    for (int u = 0; u < kRlePer; u++) R.bits[u] = w0 + u < words ? P[w0 + u] : 0u;  // all loads first
+    R.last = -1;
+#pragma unroll
+    for (int u = 0; u < kRlePer; u++) {
+        const uint64_t w = w0 + u;
+        const uint32_t bits = R.bits[u];
+        const uint32_t prev = u ? (R.bits[u - 1] & 1u) : (w0 ? (before & 1u) : ((~bits >> 31) & 1u));
+        uint32_t B = bits ^ ((bits >> 1) | (prev << 31));
+        if (w >= words) {
+            B = 0u;
+        } else {
+            const uint64_t left = n - w * 32;
+            if (left < 32) B &= ~(0xFFFFFFFFu >> left);  // keep the first `left` positions
+        }
+        R.B[u] = B;
+        if (B) R.last = (int64_t)(w * 32 + 32 - (uint64_t)__ffs(B));
+    }
+}
+
+// Pass 1: last change position of each tile.
+__global__ void __launch_bounds__(256) rle_tile_last_kernel(const uint32_t* __restrict__ P, uint64_t words, uint64_t n,
+                                                            int64_t* __restrict__ tile_last) {
+    RleWords R;
+    rle_words(P, words, n, blockIdx.x * kRleTile + (uint64_t)threadIdx.x * kRlePer, R);
+    uint64_t total;
+    block_excl_scan<MaxOp>((uint64_t)R.last, &total);
+    if (threadIdx.x == 0) tile_last[blockIdx.x] = (int64_t)total;
+}
+
+// Pass 3: with the run start in force at each tile (exclusive max-scan of
+// pass 1), the pairs of each tile; each thread keeps the run start in force at
+// its first word and its pair offset inside the tile for pass 5.
+__global__ void __launch_bounds__(256) rle_tile_count_kernel(const uint32_t* __restrict__ P, uint64_t words, uint64_t n,
+                                                             const int64_t* __restrict__ tile_s_in,
+                                                             uint32_t* __restrict__ tile_cnt,
+                                                             int64_t* __restrict__ thread_s,
+                                                             uint32_t* __restrict__ thread_off) {
+    const uint64_t w0 = blockIdx.x * kRleTile + (uint64_t)threadIdx.x * kRlePer;
+    RleWords R;
+    rle_words(P, words, n, w0, R);
+    uint64_t total;
+    const int64_t ex = (int64_t)block_excl_scan<MaxOp>((uint64_t)R.last, &total);
+    const int64_t s_in = tile_s_in[blockIdx.x] > ex ? tile_s_in[blockIdx.x] : ex;
+    int64_t s = s_in;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int u = 0; u < kRlePer; u++) {
+        const uint64_t w = w0 + u;
+        if (w >= words) break;
+        const int64_t end = (int64_t)((n - w * 32) < 32 ? n : w * 32 + 32);
+        cnt += word_pairs<false>(R.bits[u], R.B[u], (int64_t)(w * 32), end, s, 0, nullptr, nullptr);
+    }
+    uint64_t tile_total;
+    const uint64_t o0 = block_excl_scan<SumOp>(cnt, &tile_total);
+    const uint64_t g = blockIdx.x * 256ull + threadIdx.x;
+    thread_s[g] = s_in;
+    thread_off[g] = (uint32_t)o0;
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = (uint32_t)tile_total;
+}
+
+// Pass 5: each thread emits the pairs of its words from its tile's offset
+// (exclusive sum-scan of the counts) plus its offset inside the tile.
+__global__ void __launch_bounds__(256) rle_tile_emit_kernel(const uint32_t* __restrict__ P, uint64_t words, uint64_t n,
+                                                            const int64_t* __restrict__ thread_s,
+                                                            const uint32_t* __restrict__ thread_off,
+                                                            const uint64_t* __restrict__ tile_off,
+                                                            uint64_t* __restrict__ starts, uint8_t* __restrict__ out) {
+    const uint64_t w0 = blockIdx.x * kRleTile + (uint64_t)threadIdx.x * kRlePer;
+    if (w0 >= words) return;
+    RleWords R;
+    rle_words(P, words, n, w0, R);
+    const uint64_t g = blockIdx.x * 256ull + threadIdx.x;
+    int64_t s = thread_s[g];
+    uint64_t o = tile_off[blockIdx.x] + thread_off[g];
+#pragma unroll
+    for (int u = 0; u < kRlePer; u++) {
+        const uint64_t w = w0 + u;
+        if (w >= words) break;
+        const int64_t end = (int64_t)((n - w * 32) < 32 ? n : w * 32 + 32);
+        o += word_pairs<true>(R.bits[u], R.B[u], (int64_t)(w * 32), end, s, o, starts, out);
+    }
+}
+
+__global__ void __launch_bounds__(256) copy_u32_to_u64_kernel(const uint32_t* __restrict__ in,
+                                                              uint64_t* __restrict__ out, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = in[i];
 }
 
 __global__ void __launch_bounds__(256) rle_count_bytes_kernel(const uint64_t* __restrict__ starts, uint64_t np,
@@ -744,22 +823,29 @@ vox_status vox_encode(vox_encoder* e, const uint32_t* d_table, uint32_t grid, in
     const unsigned gb = grid_for(words);
 
     if (format == VOX_ENC_BINVOX) {
-        ENC_CHECK(grow(e->aux, e->aux_bytes, words * 8));
-        int64_t* last = (int64_t*)e->aux;
-        rle_last_change_kernel<<<gb, 256, 0, st>>>(P, words, n, last);
-        // run start in force at each word's first voxel: exclusive max-scan, in place
-        ENC_HIP((device_scan<MaxOp, int64_t>(last, words, sums, (uint64_t*)last, st)), "max-scan");
-        rle_pairs_kernel<false><<<gb, 256, 0, st>>>(P, words, n, (const uint64_t*)last, cnt, nullptr, nullptr, nullptr);
-        ENC_HIP((device_scan<SumOp, uint32_t>(cnt, words, sums, off, st)), "sum-scan");
+        // tiles of kRleTile words: last change per tile -> exclusive max-scan (one
+        // workgroup) -> pairs per tile -> exclusive sum-scan -> pairs emitted
+        const uint64_t nt = (words + kRleTile - 1) / kRleTile;
+        ENC_CHECK(grow(e->aux, e->aux_bytes, nt * 8 + nt * 256 * 12 + 16));
+        int64_t* tile_s = (int64_t*)e->aux;                            // last change, then run start, per tile
+        int64_t* thread_s = tile_s + nt;                               // run start at each thread's words
+        uint32_t* thread_off = (uint32_t*)(thread_s + nt * 256);      // pair offset inside the tile
+        uint32_t* tile_cnt = cnt;                                      // pairs per tile
+        uint64_t* tile_off = off;                                      // their exclusive prefix
+        rle_tile_last_kernel<<<(unsigned)nt, 256, 0, st>>>(P, words, n, tile_s);
+        scan_top_kernel<MaxOp><<<1, 256, 0, st>>>((uint64_t*)tile_s, nt);
+        rle_tile_count_kernel<<<(unsigned)nt, 256, 0, st>>>(P, words, n, tile_s, tile_cnt, thread_s, thread_off);
         e->host[1] = 0;
-        ENC_HIP(hipMemcpyAsync(e->host, off + words - 1, 8, hipMemcpyDeviceToHost, st), "read-back");
-        ENC_HIP(hipMemcpyAsync(e->host + 1, cnt + words - 1, 4, hipMemcpyDeviceToHost, st), "read-back");
+        ENC_HIP(hipMemcpyAsync(e->host + 1, tile_cnt + nt - 1, 4, hipMemcpyDeviceToHost, st), "read-back");
+        copy_u32_to_u64_kernel<<<grid_for(nt), 256, 0, st>>>(tile_cnt, tile_off, nt);
+        scan_top_kernel<SumOp><<<1, 256, 0, st>>>(tile_off, nt);
+        ENC_HIP(hipMemcpyAsync(e->host, tile_off + nt - 1, 8, hipMemcpyDeviceToHost, st), "read-back");
         ENC_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
         const uint64_t np = e->host[0] + (e->host[1] & 0xFFFFFFFFull);
         ENC_CHECK(grow(e->starts, e->starts_bytes, np * 8 + 8));
         ENC_CHECK(grow(e->out, e->out_bytes, np * 2 + 16));
-        rle_pairs_kernel<true><<<gb, 256, 0, st>>>(P, words, n, (const uint64_t*)last, nullptr, off,
-                                                   (uint64_t*)e->starts, (uint8_t*)e->out);
+        rle_tile_emit_kernel<<<(unsigned)nt, 256, 0, st>>>(P, words, n, thread_s, thread_off, tile_off,
+                                                           (uint64_t*)e->starts, (uint8_t*)e->out);
         rle_count_bytes_kernel<<<grid_for(np), 256, 0, st>>>((const uint64_t*)e->starts, np, n, (uint8_t*)e->out);
         ENC_HIP(hipGetLastError(), "binvox kernels");
         e->result = np * 2;
