@@ -192,8 +192,14 @@ struct SolidTri {
     float v0[3];      // translated vertex 0 (x used by the x-intercept)
     float A[2], B[2], C[2];  // YZ projection, counter-clockwise
     float ymin, ymax, zmin, zmax;  // integer-valued column bounds (floats)
+    uint32_t tl;      // top-left rule of edges AB, BC, CA (bits 0, 1, 2)
     bool valid;       // false when |n.x| < float_error (triangle parallel to x)
 };
+
+// check_point_triangle + top-left rule (cpu_voxelizer.cpp:196-238, :294)
+__device__ __forceinline__ bool top_left(float p0, float p1, float q0, float q1) {
+    return (q1 < p1) || (q1 == p1 && p0 > q0);
+}
 
 __device__ __forceinline__ void solid_setup(const float v[3][3], const vox_info& info, SolidTri& S) {
     float e0[3], e1[3];
@@ -226,27 +232,31 @@ __device__ __forceinline__ void solid_setup(const float v[3][3], const vox_info&
     S.zmax = floorf(mxz / info.unit[2] - 0.5f);
     S.ymin = ceilf(mny / info.unit[1] - 0.5f);
     S.zmin = ceilf(mnz / info.unit[2] - 0.5f);
+    S.tl = (top_left(A0, A1, B0, B1) ? 1u : 0u) | (top_left(B0, B1, C0, C1) ? 2u : 0u) |
+           (top_left(C0, C1, A0, A1) ? 4u : 0u);
 }
 
-// check_point_triangle + top-left rule (cpu_voxelizer.cpp:196-238, :294)
-__device__ __forceinline__ bool top_left(float p0, float p1, float q0, float q1) {
-    return (q1 < p1) || (q1 == p1 && p0 > q0);
-}
+// (double)f < 1e-6 for a float f is f < 0x1.0c6f7cp-20f, the smallest float
+// above 1e-6 (0x1.0c6f7ap-20f, the float below, is less than 1e-6).
+__device__ __forceinline__ bool below_float_error(float f) { return fabsf(f) < 0x1.0c6f7cp-20f; }
 
+// The reference tests the three edges in order and returns at the first on
+// which the point lies (cpu_voxelizer.cpp:196-238); here every test is
+// evaluated (no side effects, same operations) and the first one selected,
+// which keeps the column loops free of divergent branches.
 __device__ __forceinline__ bool column_hit(const SolidTri& S, float py, float pz) {
-    float PAx = py - S.A[0], PAy = pz - S.A[1];
-    float PBx = py - S.B[0], PBy = pz - S.B[1];
-    float PCx = py - S.C[0], PCy = pz - S.C[1];
-    float t1 = PAx * PBy - PAy * PBx;
-    if ((double)fabsf(t1) < kFloatError && PAx * PBx <= 0.0f && PAy * PBy <= 0.0f)
-        return top_left(S.A[0], S.A[1], S.B[0], S.B[1]);
-    float t2 = PBx * PCy - PBy * PCx;
-    if ((double)fabsf(t2) < kFloatError && PBx * PCx <= 0.0f && PBy * PCy <= 0.0f)
-        return top_left(S.B[0], S.B[1], S.C[0], S.C[1]);
-    float t3 = PCx * PAy - PCy * PAx;
-    if ((double)fabsf(t3) < kFloatError && PCx * PAx <= 0.0f && PCy * PAy <= 0.0f)
-        return top_left(S.C[0], S.C[1], S.A[0], S.A[1]);
-    return (t1 * t2 > 0.0f) && (t1 * t3 > 0.0f);
+    const float PAx = py - S.A[0], PAy = pz - S.A[1];
+    const float PBx = py - S.B[0], PBy = pz - S.B[1];
+    const float PCx = py - S.C[0], PCy = pz - S.C[1];
+    const float t1 = PAx * PBy - PAy * PBx;
+    const float t2 = PBx * PCy - PBy * PCx;
+    const float t3 = PCx * PAy - PCy * PAx;
+    const bool on1 = below_float_error(t1) & (PAx * PBx <= 0.0f) & (PAy * PBy <= 0.0f);
+    const bool on2 = below_float_error(t2) & (PBx * PCx <= 0.0f) & (PBy * PCy <= 0.0f);
+    const bool on3 = below_float_error(t3) & (PCx * PAx <= 0.0f) & (PCy * PAy <= 0.0f);
+    const bool inside = (t1 * t2 > 0.0f) & (t1 * t3 > 0.0f);
+    const uint32_t edge = on1 ? 1u : (on2 ? 2u : (on3 ? 4u : 0u));
+    return edge ? (S.tl & edge) != 0u : inside;
 }
 
 // x-run end of a hit column: int(float(xc / unit.x) - 0.5) with the
@@ -260,10 +270,9 @@ __device__ __forceinline__ int64_t column_xmax(const SolidTri& S, const vox_info
     float b = S.n[2] * (pz - S.v0[2]);
     float s = a + b;
     float xc = (-s) / S.n[0] + S.v0[0];
-    double q = (double)(xc / info.unit[0]) - 0.5;
-    if (!(q > -1.0)) return -1;
-    if (q >= 4.0e18) return (int64_t)4000000000000000000ll;
-    return (int64_t)q;
+    const double q = (double)(xc / info.unit[0]) - 0.5;
+    const double qc = q < 4.0e18 ? q : 4.0e18;
+    return q > -1.0 ? (int64_t)qc : (int64_t)-1;
 }
 
 // ---- index layouts (Appendix: MSB-first bits, word = idx >> 5) ---------------

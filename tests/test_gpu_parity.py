@@ -291,6 +291,31 @@ def test_solid_marker_scratch_reuse(oracle, gpu):
         assert _mismatch(_host_table(t), exp) == 0, (grid, morton, acc)
 
 
+@pytest.mark.parametrize("grid", [64, 96])
+def test_solid_deep_columns(oracle, gpu, grid):
+    """Forty nested closed shells: up to 80 crossings per column, so most
+    markers toggle a word that other runs toggle too; then a single shell on
+    the same stream (state left by the previous call)."""
+    from cuda_voxelizer_amd import meshgen
+    v, f = meshgen.icosphere_sh(level=3, seed=grid)
+    shells = [meshgen.to_aos(v * np.float32(0.2 + 0.02 * k), f) for k in range(40)]
+    print(f"deep columns G={grid}", flush=True)
+    got, exp = _run(oracle, gpu, np.concatenate(shells), grid, False, True, "marker")
+    assert _mismatch(got, exp) == 0
+    got, exp = _run(oracle, gpu, shells[-1], grid, False, True, "marker")
+    assert _mismatch(got, exp) == 0
+
+
+def test_solid_wide_triangles_1024(oracle, gpu):
+    """A coarse closed mesh (80 triangles) on a 1024^3 grid: every triangle
+    covers thousands of columns and is split over the marker workgroup."""
+    from cuda_voxelizer_amd import meshgen
+    v, f = meshgen.icosphere_sh(level=1, seed=5)
+    print("wide triangles G=1024", flush=True)
+    got, exp = _run(oracle, gpu, meshgen.to_aos(v, f), 1024, False, True, "marker")
+    assert _mismatch(got, exp) == 0
+
+
 @pytest.mark.parametrize("grid", [256, 512])
 def test_large_triangle_soup_chunk(oracle, gpu, grid):
     """Chunks whose triangles are all queued (boxes far above 256 voxels), so the
