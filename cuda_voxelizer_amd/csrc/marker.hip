@@ -321,8 +321,53 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
     const uint32_t nrows = 32u * rz;
     const uint32_t z0 = (rb / nby) * rz, y0 = (rb % nby) * 32u;
     // rows (y0.., z) for each z of the block are 32 * rw contiguous words; eight
-    // loads in flight per thread before their LDS stores
+    // loads in flight per thread before their LDS stores (16-byte loads when
+    // rows are whole quads: 32 KB of a 32 x 8 x 1024 block in flight at once)
     const uint32_t total = nrows * rw;
+    const bool linear = layout == VOX_LAYOUT_LINEAR;
+    // Morton output with rows of at most 64 quads: the suffix is taken on the
+    // loaded quads (a row's quads sit in consecutive lanes of one wave)
+    const bool quads = (rw & 3u) == 0 && (((uintptr_t)markers) & 15u) == 0;
+    const bool fused = SUFFIX && !linear && quads && rw <= 256u;
+    if (quads) {
+        const uint32_t total4 = total >> 2, rq = rw >> 2, zq = 8u * rw;  // quads: all, per row, per z
+        const uint32_t lane = (uint32_t)__lane_id(), j = lane % rq, seg0 = lane - j;
+        const uint4* __restrict__ m4 = (const uint4*)markers;
+        for (uint32_t i0 = threadIdx.x; i0 < total4; i0 += 256u * 8u) {
+            uint4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const uint32_t i = i0 + 256u * (uint32_t)u;
+                const uint32_t zl = i / zq, k = i - zl * zq;
+                v[u] = i < total4 ? m4[((((uint64_t)(z0 + zl) * G + y0) * rw) >> 2) + k] : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const uint32_t i = i0 + 256u * (uint32_t)u;
+                if (fused) {
+                    // as suffix_rows4_range: carry = parity of the row's quads of larger x
+                    uint4& q = v[u];
+                    const uint32_t pa = __popc(q.x) & 1, pb = __popc(q.y) & 1, pc = __popc(q.z) & 1,
+                                   pd = __popc(q.w) & 1;
+                    const unsigned long long par = __ballot(i < total4 && (pa ^ pb ^ pc ^ pd));
+                    const unsigned long long above =
+                        j + 1 < rq ? ((par >> (seg0 + j + 1)) & ((1ull << (rq - 1 - j)) - 1ull)) : 0ull;
+                    const uint32_t cw = (uint32_t)(__popcll(above) & 1), cz = cw ^ pd, cy = cz ^ pc, cx = cy ^ pb;
+                    q.w = prefix_xor_up(q.w) ^ (0u - cw);
+                    q.z = prefix_xor_up(q.z) ^ (0u - cz);
+                    q.y = prefix_xor_up(q.y) ^ (0u - cy);
+                    q.x = prefix_xor_up(q.x) ^ (0u - cx);
+                }
+                if (i < total4) {
+                    uint32_t* d = tile + (i / rq) * ts + (i % rq) * 4u;
+                    d[0] = v[u].x;
+                    d[1] = v[u].y;
+                    d[2] = v[u].z;
+                    d[3] = v[u].w;
+                }
+            }
+        }
+    } else
     for (uint32_t i0 = threadIdx.x; i0 < total; i0 += 256u * 8u) {
         uint32_t v[8];
 #pragma unroll
@@ -339,9 +384,8 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
     }
     __syncthreads();
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
-    const bool linear = layout == VOX_LAYOUT_LINEAR;
-    if (!SUFFIX) {
-        // rows stay as loaded
+    if (!SUFFIX || fused) {
+        // rows stay as loaded (and suffixed)
     } else if (rw <= 64) {
         const uint32_t per = 64 / rw;
         const uint32_t r_in = (uint32_t)lane / rw, j = (uint32_t)lane % rw;
@@ -574,14 +618,29 @@ __global__ void __launch_bounds__(256) bricks_to_rows_kernel(const uint32_t* __r
     const uint32_t z0 = (rb / nby) * 8u, y0 = (rb % nby) * 32u;
     const uint64_t base_w = morton3(0u, y0, z0) >> 5;
     uint8_t* tb = (uint8_t*)tile;
-    for (uint32_t t = threadIdx.x; t < G; t += 256) {
+    // all of a thread's brick loads (G / 256 <= 4 groups of 8 words) are issued first
+    uint4 ld[4][2];
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+        const uint32_t t = threadIdx.x + 256u * (uint32_t)it;
+        if (t < G) {
+            const uint32_t z2 = t & 1u, x3 = (t >> 1) & 1u, y3 = (t >> 2) & 1u, x4 = (t >> 3) & 1u,
+                           y4 = (t >> 4) & 1u, xh = t >> 5;
+            uint64_t wl = ((uint64_t)z2 << 3) | ((uint64_t)x3 << 4) | ((uint64_t)y3 << 5) | ((uint64_t)x4 << 7) |
+                          ((uint64_t)y4 << 8);
+            for (uint32_t b = 0; (xh >> b) != 0u; b++) wl |= (uint64_t)((xh >> b) & 1u) << (10u + 3u * b);
+            const uint4* s4 = (const uint4*)(src + base_w + wl);
+            ld[it][0] = s4[0];
+            ld[it][1] = s4[1];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+        const uint32_t t = threadIdx.x + 256u * (uint32_t)it;
+        if (t >= G) break;
         const uint32_t z2 = t & 1u, x3 = (t >> 1) & 1u, y3 = (t >> 2) & 1u, x4 = (t >> 3) & 1u,
                        y4 = (t >> 4) & 1u, xh = t >> 5;
-        uint64_t wl = ((uint64_t)z2 << 3) | ((uint64_t)x3 << 4) | ((uint64_t)y3 << 5) | ((uint64_t)x4 << 7) |
-                      ((uint64_t)y4 << 8);
-        for (uint32_t b = 0; (xh >> b) != 0u; b++) wl |= (uint64_t)((xh >> b) & 1u) << (10u + 3u * b);
-        const uint4* s4 = (const uint4*)(src + base_w + wl);
-        const uint4 lo = s4[0], hi = s4[1];
+        const uint4 lo = ld[it][0], hi = ld[it][1];
         const uint32_t out[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
         uint32_t W[8];
 #pragma unroll
@@ -609,6 +668,17 @@ __global__ void __launch_bounds__(256) bricks_to_rows_kernel(const uint32_t* __r
     }
     __syncthreads();
     const uint32_t total = 256u * rw;
+    if ((rw & 3u) == 0) {
+        // 16-byte stores: a row is rw / 4 whole quads
+        const uint32_t rq = rw >> 2, zq = 8u * rw;
+        uint4* __restrict__ d4 = (uint4*)dst;
+        for (uint32_t i = threadIdx.x; i < (total >> 2); i += 256) {
+            const uint32_t zl = i / zq, k = i - zl * zq;
+            const uint32_t* r = tile + (i / rq) * ts + (i % rq) * 4u;
+            d4[((((uint64_t)(z0 + zl) * G + y0) * rw) >> 2) + k] = make_uint4(r[0], r[1], r[2], r[3]);
+        }
+        return;
+    }
     for (uint32_t i = threadIdx.x; i < total; i += 256) {
         const uint32_t zl = i / (32u * rw), k = i % (32u * rw);
         dst[((uint64_t)(z0 + zl) * G + y0) * rw + k] = tile[(i / rw) * ts + i % rw];

@@ -279,7 +279,7 @@ def test_solid_marker_scratch_reuse(oracle, gpu):
     v, f = meshgen.icosphere_sh(level=4, seed=3)
     tris = meshgen.to_aos(v, f)
     for grid, morton, acc in [(128, False, False), (64, True, False), (96, False, True), (128, True, True),
-                              (32, False, False), (128, False, False)]:
+                              (256, False, True), (256, True, True), (32, False, False), (128, False, False)]:
         info = V.voxinfo_for_mesh(tris.reshape(-1, 3), grid, tris.shape[0])
         d = _dev_tris(tris, gpu)
         base = np.random.default_rng(grid).integers(0, 2**32, V.table_words(grid), dtype=np.uint64).astype(np.uint32)
