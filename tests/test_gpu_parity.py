@@ -249,6 +249,34 @@ def test_solid_runs_leaving_the_row_gpu(oracle, gpu, algo, morton):
     assert _mismatch(got, exp) == 0
 
 
+@pytest.mark.parametrize("algo", SOLID_ALGOS)
+@pytest.mark.parametrize("morton", [False, True])
+def test_solid_edges_through_column_centres(oracle, gpu, algo, morton):
+    """Quads with corners on column centres (unit voxels, half-integer y and z),
+    each split along a diagonal into two triangles: many columns lie exactly on
+    a horizontal, vertical or shared diagonal edge, or on a vertex, so the
+    reference's edge order and top-left rule decide them."""
+    import cuda_voxelizer_amd as V
+    rng = np.random.default_rng(11)
+    tris = []
+    for _ in range(200):
+        y0, z0 = (int(c) for c in rng.integers(0, 27, 2))
+        w, h = (int(c) for c in rng.integers(1, 5, 2))
+        x = rng.uniform(1.0, 31.0, 4)
+        a, b = (y0 + 0.5, z0 + 0.5), (y0 + w + 0.5, z0 + 0.5)
+        c, d = (y0 + w + 0.5, z0 + h + 0.5), (y0 + 0.5, z0 + h + 0.5)
+        if rng.integers(0, 2):
+            tris += [[[x[0], *a], [x[1], *b], [x[2], *c]], [[x[0], *a], [x[2], *c], [x[3], *d]]]
+        else:
+            tris += [[[x[1], *b], [x[2], *c], [x[3], *d]], [[x[1], *b], [x[3], *d], [x[0], *a]]]
+    tris = np.asarray(tris, np.float32)
+    box = V.AABox(np.zeros(3, np.float32), np.full(3, 32, np.float32))
+    info = V.VoxInfo(box, 32, tris.shape[0])
+    got, exp = _run(oracle, gpu, tris, 32, morton, True, algo, info=info)
+    assert _mismatch(got, exp) == 0
+    assert np.unpackbits(got.view(np.uint8)).any()
+
+
 @pytest.mark.parametrize("algo", ["marker", "tile"])
 @pytest.mark.parametrize("grid,level", [(64, 1), (128, 2), (256, 5)])
 def test_solid_coarse_meshes_both_layouts(oracle, gpu, algo, grid, level):
