@@ -325,8 +325,9 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
     // rows are whole quads: 32 KB of a 32 x 8 x 1024 block in flight at once)
     const uint32_t total = nrows * rw;
     const bool linear = layout == VOX_LAYOUT_LINEAR;
-    // Morton output with rows of at most 64 quads: the suffix is taken on the
-    // loaded quads (a row's quads sit in consecutive lanes of one wave)
+    // Morton output (power-of-two G) with rows of at most 64 quads: the suffix
+    // is taken on the loaded quads (a row's rw / 4 quads sit in consecutive
+    // lanes of one wave, and every wave runs the same number of iterations)
     const bool quads = (rw & 3u) == 0 && (((uintptr_t)markers) & 15u) == 0;
     const bool fused = SUFFIX && !linear && quads && rw <= 256u;
     if (quads) {
@@ -367,19 +368,20 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
                 }
             }
         }
-    } else
-    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += 256u * 8u) {
-        uint32_t v[8];
+    } else {
+        for (uint32_t i0 = threadIdx.x; i0 < total; i0 += 256u * 8u) {
+            uint32_t v[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const uint32_t i = i0 + 256u * (uint32_t)u;
-            const uint32_t zl = i / (32u * rw), k = i % (32u * rw);
-            v[u] = i < total ? markers[((uint64_t)(z0 + zl) * G + y0) * rw + k] : 0u;
-        }
+            for (int u = 0; u < 8; u++) {
+                const uint32_t i = i0 + 256u * (uint32_t)u;
+                const uint32_t zl = i / (32u * rw), k = i % (32u * rw);
+                v[u] = i < total ? markers[((uint64_t)(z0 + zl) * G + y0) * rw + k] : 0u;
+            }
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const uint32_t i = i0 + 256u * (uint32_t)u;
-            if (i < total) tile[(i / rw) * ts + i % rw] = v[u];
+            for (int u = 0; u < 8; u++) {
+                const uint32_t i = i0 + 256u * (uint32_t)u;
+                if (i < total) tile[(i / rw) * ts + i % rw] = v[u];
+            }
         }
     }
     __syncthreads();
