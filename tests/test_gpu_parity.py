@@ -226,6 +226,29 @@ def test_solid_row_widths(oracle, gpu, grid):
     assert np.unpackbits(got.view(np.uint8)).any()
 
 
+def test_solid_morton_2048_small_sphere(oracle, gpu):
+    """Morton solid at 2048^3: the marker table is assembled into bricks from
+    blocks of 32 x 2 rows of 16 quads each (the row suffix taken on the loaded
+    quads).  A small closed mesh in a large box keeps the oracle's runs short;
+    the 1 GiB tables are compared word for word."""
+    import torch
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import meshgen
+    v, f = meshgen.icosphere_sh(level=3, seed=2048)
+    v = (v * np.float32(0.04) + np.float32(0.3)).astype(np.float32)
+    tris = meshgen.to_aos(v, f)
+    box = V.AABox(np.array([-1, -1, -1], np.float32), np.array([1, 1, 1], np.float32))
+    info = V.VoxInfo(box, 2048, f.shape[0])
+    print("solid Morton G=2048", flush=True)
+    t = V.voxelize_solid_table(info, _dev_tris(tris, gpu), morton=True, algo="marker")
+    got = _host_table(t)
+    del t
+    torch.cuda.empty_cache()
+    exp, _, _ = oracle.solid(_oracle_info(oracle, info), tris, morton=True)
+    assert np.array_equal(got, exp)
+    assert got.any()
+
+
 @pytest.mark.parametrize("algo", SOLID_ALGOS)
 @pytest.mark.parametrize("morton", [False, True])
 def test_solid_runs_leaving_the_row_gpu(oracle, gpu, algo, morton):
