@@ -45,6 +45,7 @@ constexpr uint32_t kSpillCap = 1u << 16;
 constexpr int kJobShift = 42;  // spill records: job index above the bit index (G^3 <= 2^42)
 constexpr unsigned long long kBitMask = (1ull << kJobShift) - 1ull;
 constexpr uint32_t kSuffixRows = 256;  // rows per work unit of the job-mapped suffix
+constexpr int kSpillBlocks = 64;      // spill pass workgroups (each takes whole records)
 
 // misc words: [0] spill count, [1] spill overflow flag, [2] spill blocks done.
 // Zero when allocated; the spill pass returns them to zero, so every call
@@ -788,7 +789,7 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
                                                                                  accumulate ? 1 : 0);
     }
     if (linear) {
-        spill_kernel<<<256, 256, 0, stream>>>(fin, none, none, ws);
+        spill_kernel<<<kSpillBlocks, 256, 0, stream>>>(fin, none, none, ws);
     }
     return hipGetLastError();
 }
@@ -811,7 +812,7 @@ hipError_t launch_marker_jobs(const ItemMap& m_blocks, const ItemMap& m_rows, co
     const ItemDev dummy{};
     if (blocks) solid_marker_kernel<true><<<blocks, kMBlock, 0, stream>>>(dummy, m_blocks, ws, 1);
     if (row_units) suffix_jobs_kernel<<<row_units, 256, 0, stream>>>(m_rows);
-    spill_kernel<<<256, 256, 0, stream>>>(dummy, m_blocks, m_tris, ws);
+    spill_kernel<<<kSpillBlocks, 256, 0, stream>>>(dummy, m_blocks, m_tris, ws);
     return hipGetLastError();
 }
 
