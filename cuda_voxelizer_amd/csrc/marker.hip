@@ -311,10 +311,11 @@ __global__ void __launch_bounds__(256) suffix_jobs_kernel(ItemMap M) {
 // brick row and emits the 8 Morton words of its 8 groups of 4 x.
 // SUFFIX = false: the rows are copied as they are (linear -> Morton layout
 // conversion of a finished table).
+// SUFFIX = true: `markers` is the library's scratch marker table, and the block
+// returns its rows to zero once loaded, so the next call needs no zero-fill.
 template <bool SUFFIX>
-__global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __restrict__ markers,
-                                                           uint32_t* __restrict__ table, uint32_t G, uint32_t rz,
-                                                           int layout, int accumulate) {
+__global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* markers, uint32_t* __restrict__ table,
+                                                           uint32_t G, uint32_t rz, int layout, int accumulate) {
     extern __shared__ uint32_t tile[];  // [rz * 32 rows][rw + 1]: the pad word spreads rows over the banks
     const uint32_t rw = G >> 5, nby = G >> 5;
     const uint32_t ts = rw + 1;  // row stride in LDS
@@ -342,6 +343,15 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
                 const uint32_t i = i0 + 256u * (uint32_t)u;
                 const uint32_t zl = i / zq, k = i - zl * zq;
                 v[u] = i < total4 ? m4[((((uint64_t)(z0 + zl) * G + y0) * rw) >> 2) + k] : make_uint4(0u, 0u, 0u, 0u);
+            }
+            if (SUFFIX) {
+                uint4* c4 = const_cast<uint4*>(m4);
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const uint32_t i = i0 + 256u * (uint32_t)u;
+                    const uint32_t zl = i / zq, k = i - zl * zq;
+                    if (i < total4) c4[((((uint64_t)(z0 + zl) * G + y0) * rw) >> 2) + k] = make_uint4(0u, 0u, 0u, 0u);
+                }
             }
 #pragma unroll
             for (int u = 0; u < 8; u++) {
@@ -377,6 +387,15 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* __res
                 const uint32_t i = i0 + 256u * (uint32_t)u;
                 const uint32_t zl = i / (32u * rw), k = i % (32u * rw);
                 v[u] = i < total ? markers[((uint64_t)(z0 + zl) * G + y0) * rw + k] : 0u;
+            }
+            if (SUFFIX) {
+                uint32_t* c = const_cast<uint32_t*>(markers);
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const uint32_t i = i0 + 256u * (uint32_t)u;
+                    const uint32_t zl = i / (32u * rw), k = i % (32u * rw);
+                    if (i < total) c[((uint64_t)(z0 + zl) * G + y0) * rw + k] = 0u;
+                }
             }
 #pragma unroll
             for (int u = 0; u < 8; u++) {
@@ -707,6 +726,7 @@ struct MarkerBuffers {
     void* ws = nullptr;       // misc + spill records
     void* scratch = nullptr;  // linear marker table (Morton output, accumulation)
     uint64_t scratch_bytes = 0;
+    bool scratch_dirty = true;  // false once zero: the brick pass returns it to zero
 };
 static std::mutex g_mws_mutex;
 static std::map<std::pair<int, hipStream_t>, MarkerBuffers> g_mws;
@@ -728,6 +748,7 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
     if (e != hipSuccess) return e;
     MarkerWs ws;
     uint32_t* markers = table;
+    bool need_fill = in_place;
     {
         std::lock_guard<std::mutex> lock(g_mws_mutex);
         MarkerBuffers& b = g_mws[{dev, stream}];
@@ -747,12 +768,19 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
             e = hipMalloc(&b.scratch, table_bytes);
             if (e != hipSuccess) return e;
             b.scratch_bytes = table_bytes;
+            b.scratch_dirty = true;
         }
         ws = MarkerWs{(unsigned*)b.ws, (unsigned long long*)((char*)b.ws + 256)};
-        if (!in_place) markers = (uint32_t*)b.scratch;
+        if (!in_place) {
+            markers = (uint32_t*)b.scratch;
+            need_fill = b.scratch_dirty;
+            b.scratch_dirty = true;  // until this call's brick pass is queued
+        }
     }
-    e = launch_fill_zero(info, markers, stream);
-    if (e != hipSuccess) return e;
+    if (need_fill) {
+        e = launch_fill_zero(info, markers, stream);
+        if (e != hipSuccess) return e;
+    }
     const bool linear = layout == VOX_LAYOUT_LINEAR;
     const ItemMap none{nullptr, nullptr, 1, 0};
     ItemDev one{};  // markers: into `markers`
@@ -787,6 +815,10 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
         const uint32_t nblocks = (G / 32u) * (G / rz);
         marker_build_kernel<true><<<nblocks, 256, 4ull * 32u * rz * (G / 32u + 1u), stream>>>(markers, table, G, rz, layout,
                                                                                  accumulate ? 1 : 0);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        std::lock_guard<std::mutex> lock(g_mws_mutex);
+        g_mws[{dev, stream}].scratch_dirty = false;  // every row of the G^3 table was read and cleared
     }
     if (linear) {
         spill_kernel<<<kSpillBlocks, 256, 0, stream>>>(fin, none, none, ws);
