@@ -12,7 +12,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <utility>
 #include <string>
 
 #include "../../include/cuda_voxelizer_amd/vox_capi.h"
@@ -31,18 +33,15 @@ __device__ __forceinline__ float key_float(uint32_t k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
 }
 
-__global__ void __launch_bounds__(256) bbox_init_kernel(uint32_t* keys) {
-    if (threadIdx.x < 3) keys[threadIdx.x] = 0xFFFFFFFFu;       // min
-    if (threadIdx.x >= 3 && threadIdx.x < 6) keys[threadIdx.x] = 0u;  // max
-}
-
 // Grid-stride min/max keys, reduced over the wave (shuffles) and the block
-// (LDS) before one atomicMin/atomicMax per key and block: at most kBboxBlocks
-// atomics per address (one per wave cost ~8K same-address atomics for a
-// 512K-vertex mesh, serialised at the memory side).
+// (LDS) into the block's six partial keys; the last block to finish (a
+// counter) reduces the partials, writes the six floats and returns the
+// counter to zero, so one launch does it all.  ws: [0] counter, [64..]
+// kBboxBlocks x 6 partial keys.
 constexpr unsigned kBboxBlocks = 64;
 
-__global__ void __launch_bounds__(256) bbox_kernel(const float* __restrict__ v, uint64_t n, uint32_t* keys) {
+__global__ void __launch_bounds__(256) bbox_kernel(const float* __restrict__ v, uint64_t n, uint32_t* ws,
+                                                   float* __restrict__ out) {
     __shared__ uint32_t part[4][6];
     uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
     // eight vertices per step, their 24 loads issued before any is used
@@ -82,17 +81,35 @@ __global__ void __launch_bounds__(256) bbox_kernel(const float* __restrict__ v, 
         }
     }
     __syncthreads();
+    __shared__ bool last;
     if (threadIdx.x < 6) {
         const int k = threadIdx.x;
         uint32_t r = part[0][k];
         for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = k < 3 ? (part[w][k] < r ? part[w][k] : r)
                                                                   : (part[w][k] > r ? part[w][k] : r);
-        if (k < 3) atomicMin(keys + k, r); else atomicMax(keys + k, r);
+        ws[64 + blockIdx.x * 6 + k] = r;
+        __threadfence();
     }
-}
-
-__global__ void __launch_bounds__(64) bbox_finish_kernel(const uint32_t* keys, float* out) {
-    if (threadIdx.x < 6) out[threadIdx.x] = key_float(keys[threadIdx.x]);
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(ws, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (last) {  // uniform over the block
+        __shared__ uint32_t allp[kBboxBlocks * 6];
+        __threadfence();
+        const volatile uint32_t* pk = ws + 64;
+        for (unsigned t = threadIdx.x; t < gridDim.x * 6u; t += blockDim.x) allp[t] = pk[t];
+        __syncthreads();
+        if (threadIdx.x < 6) {
+            const int k = threadIdx.x;
+            uint32_t r = allp[k];
+            for (unsigned b = 1; b < gridDim.x; b++) {
+                const uint32_t x = allp[b * 6 + k];
+                r = k < 3 ? (x < r ? x : r) : (x > r ? x : r);
+            }
+            out[k] = key_float(r);
+            if (k == 0) ws[0] = 0u;
+        }
+    }
 }
 
 // One workgroup per 256 faces: indices and triangles pass through LDS so the
@@ -192,6 +209,8 @@ static unsigned grid_for(uint64_t n) {
 }
 
 // pinned staging for vox_upload_async (double buffered, per process)
+static std::mutex g_bbox_mutex;
+static std::map<std::pair<int, hipStream_t>, void*> g_bbox_ws;
 static std::mutex g_up_mutex;
 static void* g_up_buf[2] = {nullptr, nullptr};
 static hipEvent_t g_up_ev[2] = {nullptr, nullptr};
@@ -207,12 +226,29 @@ vox_status vox_mesh_bbox(const float* d_vertices, uint64_t n_vertices, float* d_
     if (!d_out6 || (n_vertices && !d_vertices)) return fail(VOX_ERR_INVALID, "NULL argument");
     if (n_vertices == 0) return fail(VOX_ERR_INVALID, "empty vertex array");
     hipStream_t st = (hipStream_t)stream;
-    // the 6 keys live in the output buffer itself until the finish kernel converts them
-    uint32_t* keys = (uint32_t*)d_out6;
-    bbox_init_kernel<<<1, 64, 0, st>>>(keys);
+    uint32_t* ws = nullptr;
+    {
+        // per (device, stream): the counter and the partial keys; the counter
+        // starts at zero and the kernel's last block returns it there
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+        std::lock_guard<std::mutex> lock(g_bbox_mutex);
+        void*& w = g_bbox_ws[{dev, st}];
+        if (!w) {
+            e = hipMalloc(&w, 4u * (64u + 6u * kBboxBlocks));
+            if (e == hipSuccess) e = hipMemset(w, 0, 4u * 64u);
+            if (e == hipSuccess) e = hipDeviceSynchronize();
+            if (e != hipSuccess) {
+                if (w) (void)hipFree(w);
+                w = nullptr;
+                return hip_fail(e, "bbox workspace");
+            }
+        }
+        ws = (uint32_t*)w;
+    }
     const unsigned nb = grid_for(n_vertices) < kBboxBlocks ? grid_for(n_vertices) : kBboxBlocks;
-    bbox_kernel<<<nb, 256, 0, st>>>(d_vertices, n_vertices, keys);
-    bbox_finish_kernel<<<1, 64, 0, st>>>(keys, d_out6);
+    bbox_kernel<<<nb, 256, 0, st>>>(d_vertices, n_vertices, ws, d_out6);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? VOX_OK : hip_fail(e, "bbox kernels");
 }
