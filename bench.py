@@ -1,11 +1,15 @@
 """Voxelization throughput on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--config torus512m|...] [--algo tile|direct]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config torus1024|...] [--extra a,b] [--algo ...]
 
 A step is one voxelization of the whole workload (the table is fully defined
-by the call: zero-fill fused, mesh already resident in HBM).  Default
-workload: the 512^3 Morton surface voxelization of a jittered 1024x512-quad
-torus (1,048,576 FP32 triangles, 10% slivers).  Prints ONE JSON line.
+by the call: zero-fill included, mesh already resident in HBM).  Default
+workload: the 1024^3 linear surface voxelization of a jittered 1024x512-quad
+torus (1,048,576 FP32 triangles, 10% slivers).  The same JSON line carries,
+under "extra_workloads", the 1024^3 linear solid voxelization of the
+SH-displaced level-8 icosphere (1,310,720 triangles) and the 512^3 Morton
+surface voxelization of the torus, each with its own roofline entry, CPU
+baseline and bit check.  Prints ONE JSON line.
 
 Roofline: algorithmic bytes per step B = 36*N_tri + G^3/8 (read every triangle
 once, write the table once) over the step's device time, measured with HIP
@@ -56,19 +60,49 @@ def build_mesh(kind: str):
 
 
 def load_traffic(workload: str):
-    p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    """HBM bytes per call from the PMC runs summarised in profiles/ (FETCH_SIZE
+    and WRITE_SIZE scaled by the copy calibration); (None, None) when absent."""
+    rel = os.path.join("profiles", f"traffic_{workload}.json")
+    p = os.path.join(ROOT, rel)
     if os.path.exists(p):
         with open(p) as fh:
-            return json.load(fh).get("hbm_bytes_per_launch")
-    return None
+            return json.load(fh).get("hbm_bytes_per_launch"), rel
+    return None, None
 
 
-def cpu_baseline_leg(tris, info, grid, morton, solid, gpu_table, budget_s=20.0):
-    """Times the oracle's reference-structured path on a bounded sample and
-    checks the GPU table against the oracle (fast mode) on the full workload."""
+def host_cpu() -> dict:
+    """The host's cores as this process sees them: nproc, the CPUs it may run
+    on, the OpenMP thread allotment and the lscpu model name."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = affinity
+    if omp and omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    return {"nproc": os.cpu_count(), "affinity": affinity, "omp_num_threads": omp, "threads": threads,
+            "model": model}
+
+
+def cpu_baseline_leg(tris, info, grid, morton, solid, gpu_table, budget_s=8.0):
+    """Times the oracle's reference-structured path (one global critical
+    section per bit, cpu_voxelizer.cpp:7-15,53) on a bounded sample with every
+    core this process is allotted, median of 3 runs (1 when a run exceeds 60 s),
+    and checks the GPU table against the oracle (fast mode) on the full
+    workload."""
     from oracle import oracle as O
-    cores = os.cpu_count() or 1
-    cores = min(cores, 16)
+    host = host_cpu()
+    cores = host["threads"]
     O.set_threads(cores)
     oinfo = O.info_from_bytes(info.as_bytes())
     # bit check on the full workload
@@ -77,25 +111,100 @@ def cpu_baseline_leg(tris, info, grid, morton, solid, gpu_table, budget_s=20.0):
     else:
         exp, _ = O.surface(oinfo, tris, morton=morton, mode=O.MODE_FAST)
     mismatches = int(np.unpackbits((exp ^ gpu_table).view(np.uint8)).sum())
-    # timed reference-structured run: grow the sample until ~budget_s of work
-    n = tris.shape[0]
-    m = min(n, 65536)
-    while True:
+
+    def run(m):
         if solid:
-            _, _, ms = O.solid(oinfo, tris[:m], morton=morton, mode=O.MODE_REF)
-        else:
-            _, ms = O.surface(oinfo, tris[:m], morton=morton, mode=O.MODE_REF)
+            return O.solid(oinfo, tris[:m], morton=morton, mode=O.MODE_REF)[2]
+        return O.surface(oinfo, tris[:m], morton=morton, mode=O.MODE_REF)[1]
+
+    # grow the sample until one run holds ~budget_s / 4 of work, then time it 3 times
+    n = tris.shape[0]
+    m = min(n, 16384)
+    while True:
+        ms = run(m)
         if m == n or ms > 1000.0 * budget_s / 4:
             break
         m = min(n, m * 4)
+    runs = [run(m) for _ in range(3 if ms < 60e3 else 1)]
+    med = float(np.median(runs))
     return {
-        "value": m / (ms / 1e3),
+        "value": m / (med / 1e3),
         "unit": "triangles/s",
         "cores": cores,
         "kind": "port",
+        "host": host,
+        "runs_ms": runs,
         "sample": f"{m} of {n} triangles ({'solid' if solid else 'surface'}, {grid}^3, "
-                  f"{'morton' if morton else 'linear'}), omp critical per bit, {ms:.1f} ms",
+                  f"{'morton' if morton else 'linear'}), oracle MODE_REF (omp critical per bit), "
+                  f"median of {len(runs)} runs = {med:.1f} ms on {cores} threads",
     }, mismatches
+
+
+def mesh_sha256(tris) -> str:
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(tris, np.float32).tobytes()).hexdigest()
+
+
+def run_workload(name, args, steps, warmup, with_cpu=True) -> dict:
+    """One voxelization workload: `steps` timed calls on one stream (mesh
+    resident), HIP-event launch times, the raster kernel alone, the roofline
+    entry, and the CPU baseline + bit check."""
+    import torch
+    import cuda_voxelizer_amd as V
+    kind, grid, morton, solid, desc = CONFIGS[name]
+    tris = build_mesh(kind)
+    n = tris.shape[0]
+    dev = torch.device("cuda:0")
+    info = V.voxinfo_for_mesh(tris.reshape(-1, 3), grid, n)
+    d_tris = torch.from_numpy(tris.reshape(-1)).to(dev)
+    table = torch.empty(V.table_words(grid), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        if solid:
+            V.voxelize_solid_table(info, d_tris, morton=morton, table=table, algo=args.algo, stream=stream)
+        else:
+            V.voxelize_surface(info, d_tris, morton=morton, table=table, algo=args.algo, stream=stream)
+
+    for _ in range(warmup):
+        step()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record(stream)
+        step()
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    launch_ms = float(np.mean(ev_ms))
+    gpu_table = table.cpu().numpy().view(np.uint32).copy()
+
+    alg_bytes = 36 * n + (grid ** 3) // 8
+    achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
+    traffic, traffic_src = load_traffic(name)
+    out = {
+        "value": n * steps / wall,
+        "unit": "triangles/s",
+        "ms_per_step": wall * 1e3 / steps,
+        "grids_per_s": steps / wall,
+        "config": {"workload": desc, "name": name, "n_triangles": n, "grid": grid,
+                   "layout": "morton" if morton else "linear", "mode": "solid" if solid else "surface",
+                   "algo": args.algo, "mesh_sha256": mesh_sha256(tris)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes": alg_bytes, "launch_ms": launch_ms,
+                     "launch_ms_p50": float(np.median(ev_ms))},
+    }
+    if with_cpu:
+        cpu, mism = cpu_baseline_leg(tris, info, grid, morton, solid, gpu_table)
+        out["cpu_baseline"] = cpu
+        out["mismatches"] = mism
+    del d_tris, table
+    torch.cuda.empty_cache()
+    return out
 
 
 def main() -> None:
@@ -103,7 +212,9 @@ def main() -> None:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="torus512m", choices=sorted(CONFIGS) + list(MODES))
+    ap.add_argument("--config", default="torus1024", choices=sorted(CONFIGS) + list(MODES))
+    ap.add_argument("--extra", default="sh1024s,torus512m",
+                    help="comma-separated workloads reported as extra keys of the same line ('' for none)")
     ap.add_argument("--algo", default="auto", choices=["auto", "tile", "chunk", "direct", "marker"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -127,84 +238,26 @@ def main() -> None:
     if args.config == "rows":
         print(json.dumps(run_rows(args)), flush=True)
         return
-    kind, grid, morton, solid, desc = CONFIGS[args.config]
-    tris = build_mesh(kind)
-    n = tris.shape[0]
-    dev = torch.device("cuda:0")
-    info = V.voxinfo_for_mesh(tris.reshape(-1, 3), grid, n)
-    d_tris = torch.from_numpy(tris.reshape(-1)).to(dev)
-    table = torch.empty(V.table_words(grid), dtype=torch.int32, device=dev)
-    stream = torch.cuda.current_stream()
-
-    def step():
-        if solid:
-            V.voxelize_solid_table(info, d_tris, morton=morton, table=table, algo=args.algo, stream=stream)
-        else:
-            V.voxelize_surface(info, d_tris, morton=morton, table=table, algo=args.algo, stream=stream)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        starts[i].record(stream)
-        step()
-        ends[i].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ev_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
-    launch_ms = float(np.mean(ev_ms))
-    gpu_table = table.cpu().numpy().view(np.uint32).copy()
-    # the dominant kernel alone: a surface call that accumulates into the table
-    # launches only the raster kernel (no zero-fill); timed the same way
-    kernel_ms = None
-    if not solid:
-        for _ in range(2):
-            V.voxelize_surface(info, d_tris, morton=morton, table=table, algo=args.algo, stream=stream,
-                               accumulate=True)
-        ks = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-        ke = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-        for i in range(args.steps):
-            ks[i].record(stream)
-            V.voxelize_surface(info, d_tris, morton=morton, table=table, algo=args.algo, stream=stream,
-                               accumulate=True)
-            ke[i].record(stream)
-        torch.cuda.synchronize()
-        kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ks, ke)]))
-
-    alg_bytes = 36 * n + (grid ** 3) // 8
-    achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
-    tri_s = n * args.steps / wall
-    workload = args.config
+    main_w = run_workload(args.config, args, args.steps, args.warmup, with_cpu=not args.no_cpu_baseline)
     out = {
         "metric": "triangles/s + grids/s at zero bit mismatches",
-        "value": tri_s,
+        "value": main_w["value"],
         "unit": "triangles/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": main_w["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": desc, "name": workload, "n_triangles": n, "grid": grid,
-                   "layout": "morton" if morton else "linear", "mode": "solid" if solid else "surface",
-                   "algo": args.algo},
-        "grids_per_s": args.steps / wall,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(workload),
-                     "algorithmic_bytes": alg_bytes, "launch_ms": launch_ms,
-                     "launch_ms_p50": float(np.median(ev_ms)),
-                     "raster_kernel_ms": kernel_ms},
     }
-    if not args.no_cpu_baseline:
-        cpu, mism = cpu_baseline_leg(tris, info, grid, morton, solid, gpu_table)
-        out["cpu_baseline"] = cpu
-        out["mismatches"] = mism
+    out.update({k: v for k, v in main_w.items() if k not in ("value", "unit", "ms_per_step")})
+    extras = [w for w in args.extra.split(",") if w and w != args.config] if args.extra else []
+    if extras:
+        out["extra_workloads"] = {w: run_workload(w, args, min(args.steps, 20), min(args.warmup, 3),
+                                                  with_cpu=not args.no_cpu_baseline) for w in extras}
     print(json.dumps(out), flush=True)
 
 
@@ -512,8 +565,7 @@ def run_serve(args) -> dict:
     def run_server(reqs):
         server = VoxServer(budget_bytes=64 << 30, streams=2, max_batch=64)
         for r in pool:  # meshes resident (with their boxes) before the clock starts
-            server.meshes[id(r)] = (torch.from_numpy(r.tris.reshape(-1)).to("cuda:0"), r.tris.size * 4, 0)
-            server.boxes[id(r)] = box_cache[id(r)]
+            server.preload(id(r), r.tris, box=box_cache[id(r)])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         pending = sorted(reqs, key=lambda q: q.arrival)

@@ -173,6 +173,40 @@ def _stream_ptr(stream) -> int | None:
     return int(s.cuda_stream) or None
 
 
+class _OnStream:
+    """Run a library call on `stream` (None: the current stream).
+
+    A side stream first waits for the current stream (the inputs were written
+    there), and tensors created inside the block are allocated on the side
+    stream, so the caching allocator orders their reuse after its work.
+    Results read on the host must be read after ``sync()``."""
+
+    def __init__(self, stream, device=None):
+        torch = _torch()
+        self.cur = torch.cuda.current_stream(device)
+        self.s = stream if stream is not None else self.cur
+        self.side = self.s != self.cur
+        self._ctx = torch.cuda.stream(self.s) if self.side else None
+
+    def __enter__(self):
+        if self.side:
+            self.s.wait_stream(self.cur)
+            self._ctx.__enter__()
+        return self
+
+    def __exit__(self, *exc):
+        if self.side:
+            self._ctx.__exit__(*exc)
+        return False
+
+    @property
+    def ptr(self) -> int | None:
+        return int(self.s.cuda_stream) or None
+
+    def sync(self) -> None:
+        self.s.synchronize()
+
+
 def _info_checked(info: VoxInfo, tris, table, grid_words_needed=True):
     _check_tensor(tris, "torch.float32", "triangle_data")
     if tris.numel() < 9 * info.n_triangles:
@@ -192,13 +226,14 @@ def voxelize_surface(info: VoxInfo, tris, morton: bool = False, table=None, accu
     Without ``accumulate`` the table is fully defined by this call; with it
     bits are OR-ed into the existing contents (reference semantics)."""
     torch = _torch()
-    if table is None:
-        table = torch.empty(table_words(info.grid), dtype=torch.int32, device=tris.device)
-    _info_checked(info, tris, table)
-    flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | _algo_flag(algo)
-    lib = _capi.load()
-    check(lib.vox_surface(ctypes.byref(info._c), tris.data_ptr(), table.data_ptr(),
-                          VOX_LAYOUT_MORTON if morton else VOX_LAYOUT_LINEAR, flags, _stream_ptr(stream)))
+    with _OnStream(stream, tris.device) as on:
+        if table is None:
+            table = torch.empty(table_words(info.grid), dtype=torch.int32, device=tris.device)
+        _info_checked(info, tris, table)
+        flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | _algo_flag(algo)
+        lib = _capi.load()
+        check(lib.vox_surface(ctypes.byref(info._c), tris.data_ptr(), table.data_ptr(),
+                              VOX_LAYOUT_MORTON if morton else VOX_LAYOUT_LINEAR, flags, on.ptr))
     return table
 
 
@@ -212,17 +247,18 @@ def voxelize_solid_table(info: VoxInfo, tris, morton: bool = False, table=None, 
     the table end exactly as the reference indexes them; runs with a
     negative end are empty."""
     torch = _torch()
-    if table is None:
-        table = torch.empty(table_words(info.grid), dtype=torch.int32, device=tris.device)
-    _info_checked(info, tris, table)
-    flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | _algo_flag(algo)
-    cl_ptr = None
-    if offrow is not None:
-        _check_tensor(offrow, "torch.int64", "offrow")
-        cl_ptr = offrow.data_ptr()
-    lib = _capi.load()
-    check(lib.vox_solid(ctypes.byref(info._c), tris.data_ptr(), table.data_ptr(),
-                        VOX_LAYOUT_MORTON if morton else VOX_LAYOUT_LINEAR, flags, cl_ptr, _stream_ptr(stream)))
+    with _OnStream(stream, tris.device) as on:
+        if table is None:
+            table = torch.empty(table_words(info.grid), dtype=torch.int32, device=tris.device)
+        _info_checked(info, tris, table)
+        flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | _algo_flag(algo)
+        cl_ptr = None
+        if offrow is not None:
+            _check_tensor(offrow, "torch.int64", "offrow")
+            cl_ptr = offrow.data_ptr()
+        lib = _capi.load()
+        check(lib.vox_solid(ctypes.byref(info._c), tris.data_ptr(), table.data_ptr(),
+                            VOX_LAYOUT_MORTON if morton else VOX_LAYOUT_LINEAR, flags, cl_ptr, on.ptr))
     return table
 
 
@@ -312,9 +348,10 @@ def mesh_bbox_device(vertices, stream=None) -> AABox:
     """Device min/max of a float32 [V*3] vertex tensor (trimesh2 need_bbox)."""
     torch = _torch()
     _check_tensor(vertices, "torch.float32", "vertices")
-    out = torch.empty(6, dtype=torch.float32, device=vertices.device)
-    check(_capi.load().vox_mesh_bbox(vertices.data_ptr(), vertices.numel() // 3, out.data_ptr(),
-                                     _stream_ptr(stream)))
+    with _OnStream(stream, vertices.device) as on:
+        out = torch.empty(6, dtype=torch.float32, device=vertices.device)
+        check(_capi.load().vox_mesh_bbox(vertices.data_ptr(), vertices.numel() // 3, out.data_ptr(), on.ptr))
+    on.sync()
     h = out.cpu().numpy()
     return AABox(h[:3], h[3:])
 
@@ -326,10 +363,12 @@ def pack_triangles(vertices, faces, stream=None):
     _check_tensor(vertices, "torch.float32", "vertices")
     _check_tensor(faces, "torch.int32", "faces")
     nf = faces.numel() // 3
-    out = torch.empty(nf * 9, dtype=torch.float32, device=vertices.device)
-    bad = torch.zeros(1, dtype=torch.int64, device=vertices.device)
-    check(_capi.load().vox_pack_triangles(vertices.data_ptr(), vertices.numel() // 3, faces.data_ptr(), nf,
-                                          out.data_ptr(), bad.data_ptr(), _stream_ptr(stream)))
+    with _OnStream(stream, vertices.device) as on:
+        out = torch.empty(nf * 9, dtype=torch.float32, device=vertices.device)
+        bad = torch.zeros(1, dtype=torch.int64, device=vertices.device)
+        check(_capi.load().vox_pack_triangles(vertices.data_ptr(), vertices.numel() // 3, faces.data_ptr(), nf,
+                                              out.data_ptr(), bad.data_ptr(), on.ptr))
+    on.sync()  # the count is read here; `out` is complete for every stream after it
     if int(bad.item()):
         raise ValueError(f"{int(bad.item())} face indices out of range")
     return out
@@ -358,16 +397,18 @@ def prepare_mesh(vertices: np.ndarray, faces: np.ndarray, grid: int, device="cud
 def convert_layout(table, grid: int, to_morton: bool, stream=None):
     """Linear <-> Morton re-layout of a power-of-two grid table (new tensor)."""
     torch = _torch()
-    out = torch.empty_like(table)
-    check(_capi.load().vox_convert_layout(table.data_ptr(), out.data_ptr(), grid, int(to_morton),
-                                          _stream_ptr(stream)))
+    with _OnStream(stream, table.device) as on:
+        out = torch.empty_like(table)
+        check(_capi.load().vox_convert_layout(table.data_ptr(), out.data_ptr(), grid, int(to_morton), on.ptr))
     return out
 
 
 def count_voxels(table, stream=None) -> int:
     torch = _torch()
-    c = torch.zeros(1, dtype=torch.int64, device=table.device)
-    check(_capi.load().vox_count_voxels(table.data_ptr(), table.numel(), c.data_ptr(), _stream_ptr(stream)))
+    with _OnStream(stream, table.device) as on:
+        c = torch.zeros(1, dtype=torch.int64, device=table.device)
+        check(_capi.load().vox_count_voxels(table.data_ptr(), table.numel(), c.data_ptr(), on.ptr))
+    on.sync()
     return int(c.item())
 
 

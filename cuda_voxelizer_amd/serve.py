@@ -27,7 +27,7 @@ from __future__ import annotations
 import collections
 import ctypes
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -80,7 +80,15 @@ class _InFlight:
     event: object
     launched: float
     runner: int
-    keepalive: list = field(default_factory=list)
+
+
+@dataclass
+class _Mesh:
+    tensor: object              # float32 device tensor (9 * n_triangles)
+    nbytes: int
+    users: int                  # admitted or in-flight requests reading it
+    ready: object               # event recorded after its upload
+    host: object = None         # pinned source, kept until the upload has run
 
 
 class _Runner:
@@ -115,7 +123,8 @@ class VoxServer:
         self.queue: collections.deque[Request] = collections.deque()
         self.admitted: collections.deque[_Admitted] = collections.deque()
         self.inflight: list[_InFlight] = []
-        self.meshes: collections.OrderedDict = collections.OrderedDict()  # key -> (tensor, nbytes, users)
+        self.copy_stream = torch.cuda.Stream(device=self.device)
+        self.meshes: collections.OrderedDict = collections.OrderedDict()  # key -> _Mesh (LRU order)
         self.boxes: dict = {}  # key -> cube bounding box (createMeshBBCube of the mesh bbox)
         self.used = 0
         self.rejected: list[int] = []
@@ -130,6 +139,18 @@ class VoxServer:
         self._admit()
         self._launch()
         return out
+
+    def preload(self, mesh_key, tris: np.ndarray, box=None) -> None:
+        """Make a mesh resident before its requests arrive (counted against the
+        budget like any resident mesh; evictable once idle)."""
+        if mesh_key in self.meshes:
+            return
+        req = Request(-1, mesh_key, tris, 0, False)
+        self._mesh(req)
+        self.meshes[mesh_key].users = 0
+        self.used += self._mesh_bytes(req)
+        if box is not None:
+            self.boxes[mesh_key] = box
 
     def pending(self) -> int:
         return len(self.queue) + len(self.admitted) + sum(len(f.jobs) for f in self.inflight)
@@ -162,33 +183,36 @@ class VoxServer:
         return int(req.tris.size) * 4
 
     def _mesh(self, req: Request):
+        """Device copy of the request's mesh: uploaded once on the copy stream
+        and kept resident.  Every launch that reads it waits on its ready
+        event, whichever runner it lands on."""
         torch = self.torch
         if req.mesh_key in self.meshes:
-            t, nb, users = self.meshes.pop(req.mesh_key)
-            self.meshes[req.mesh_key] = (t, nb, users + 1)
+            m = self.meshes.pop(req.mesh_key)
+            m.users += 1
+            self.meshes[req.mesh_key] = m
             self.stats["mesh_hits"] += 1
-            return t
+            return m.tensor
         host = torch.from_numpy(np.ascontiguousarray(req.tris, np.float32).reshape(-1)).pin_memory()
-        runner = self.runners[self._next_runner]
-        with torch.cuda.stream(runner.stream):
+        with torch.cuda.stream(self.copy_stream):
             t = host.to(self.device, non_blocking=True)
-        # the pinned source must live until the copy ran: the stream's work is ordered,
-        # and the copy is recorded on the runner that will launch this job next
-        self._pinned_keep = getattr(self, "_pinned_keep", [])
-        self._pinned_keep.append((host, runner.stream))
-        nb = self._mesh_bytes(req)
-        self.meshes[req.mesh_key] = (t, nb, 1)
+            ready = torch.cuda.Event()
+            ready.record(self.copy_stream)
+        # the pinned source lives until the copy has run (dropped in _collect)
+        self.meshes[req.mesh_key] = _Mesh(t, self._mesh_bytes(req), 1, ready, host)
         self.stats["mesh_uploads"] += 1
         return t
 
-    def _evict(self, need: int) -> None:
+    def _evict(self, need: int, keep=None) -> None:
+        """Drop idle resident meshes (LRU first) until `need` more bytes fit;
+        the mesh `keep` (the one the request being admitted reads) stays."""
         for key in list(self.meshes):
             if self.used + need <= self.budget:
                 return
-            t, nb, users = self.meshes[key]
-            if users == 0:
+            m = self.meshes[key]
+            if m.users == 0 and key != keep:
                 del self.meshes[key]
-                self.used -= nb
+                self.used -= m.nbytes
 
     def _info(self, req: Request):
         """voxinfo of a request; the mesh's cube box is computed once per mesh."""
@@ -208,7 +232,7 @@ class VoxServer:
                 self.rejected.append(req.rid)
                 continue
             if self.used + need > self.budget:
-                self._evict(need)
+                self._evict(need, keep=req.mesh_key)
                 if self.used + need > self.budget:
                     return  # wait for tables to be released / batches to finish
             self.queue.popleft()
@@ -246,12 +270,13 @@ class VoxServer:
                 it.solid = 1 if a.req.solid else 0
                 it.flags = 0
                 items.append(it)
-            # meshes uploaded on another runner's stream must have landed first
-            ev_in = torch.cuda.Event()
-            for (_h, s) in getattr(self, "_pinned_keep", []):
-                if s is not runner.stream:
-                    ev_in.record(s)
-                    runner.stream.wait_event(ev_in)
+            # every mesh of the micro-batch must have landed, whichever stream uploaded it
+            waited = set()
+            for a in jobs:
+                m = self.meshes.get(a.req.mesh_key)
+                if m is not None and m.ready is not None and a.req.mesh_key not in waited:
+                    runner.stream.wait_event(m.ready)
+                    waited.add(a.req.mesh_key)
             for a in jobs:  # the caching allocator must not recycle these while the runner uses them
                 a.table.record_stream(runner.stream)
                 a.tris_dev.record_stream(runner.stream)
@@ -259,9 +284,7 @@ class VoxServer:
             ev = torch.cuda.Event()
             ev.record(runner.stream)
             runner.busy = True
-            keep = getattr(self, "_pinned_keep", [])
-            self._pinned_keep = []
-            self.inflight.append(_InFlight(jobs, ev, self.clock(), idx, keep))
+            self.inflight.append(_InFlight(jobs, ev, self.clock(), idx))
             self.stats["batches"] += 1
             self.stats["jobs"] += len(jobs)
 
@@ -273,14 +296,17 @@ class VoxServer:
                 now = self.clock()
                 self.runners[f.runner].busy = False
                 for a in f.jobs:
-                    t, nb, users = self.meshes.get(a.req.mesh_key, (None, 0, 1))
-                    if t is not None:
-                        self.meshes[a.req.mesh_key] = (t, nb, users - 1)
+                    m = self.meshes.get(a.req.mesh_key)
+                    if m is not None:
+                        m.users -= 1
                     out.append(Result(a.req.rid, a.table, a.info, a.req.arrival, a.admitted, f.launched, now,
                                       len(f.jobs)))
             else:
                 still.append(f)
         self.inflight = still
+        for m in self.meshes.values():  # finished uploads no longer need their pinned source
+            if m.host is not None and m.ready.query():
+                m.host = None
         return out
 
 

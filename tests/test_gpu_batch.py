@@ -160,3 +160,33 @@ def test_batch_chunk_work_shapes(oracle, gpu):
         b.launch()
         _check(oracle, jobs, host)
         b.close()
+
+
+def test_c4_batch_256_meshes_one_graph_launch(oracle, gpu, tmp_path):
+    """The batched workload itself: 256 meshes (seed 0xC4; triangle counts
+    log-uniform in [500, 500k], ~16M triangles in all; grids {32, 64, 128, 256};
+    icospheres and tori solid, soups surface), with the per-class algorithms
+    the autotuner picks, replayed as one graph launch; every table equals the
+    oracle's, on the first launch and on a relaunch over dirty tables."""
+    import torch
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import meshgen
+    from cuda_voxelizer_amd.autotune import Tuner
+    meshes = meshgen.batch_meshes(256, seed=0xC4)
+    jobs, host = [], []
+    for m in meshes:
+        info = V.voxinfo_for_mesh(m.tris.reshape(-1, 3), m.grid, m.tris.shape[0])
+        d = torch.from_numpy(m.tris.reshape(-1)).to(gpu)
+        t = torch.full((V.table_words(m.grid),), -1, dtype=torch.int32, device=gpu)
+        jobs.append(V.BatchJob(info, d, t, solid=m.solid))
+        host.append(m.tris)
+    Tuner(path=str(tmp_path / "autotune.json")).tune_jobs(jobs)
+    b = V.Batch(jobs, use_graph=True)
+    assert b.info()["graph"] == 1 and b.info()["jobs"] == 256
+    b.launch()
+    _check(oracle, jobs, host)
+    for j in jobs:
+        j.table.fill_(-1)
+    b.launch()
+    _check(oracle, jobs, host)
+    b.close()

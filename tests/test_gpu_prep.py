@@ -89,3 +89,37 @@ def test_bbox_and_pack_odd_sizes(gpu, nv, nf):
     np.testing.assert_array_equal(box.max, v.max(0))
     tris = V.pack_triangles(dv, df)
     np.testing.assert_array_equal(tris.cpu().numpy(), v[f].reshape(-1))
+
+
+def test_side_stream_calls_are_ordered(oracle, gpu):
+    """Every helper that takes a stream runs on it, after the caller's work on
+    the current stream, and returns results that stream produced: inputs are
+    written on the current stream immediately before each call."""
+    import torch
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import meshgen
+    side = torch.cuda.Stream()
+    v, f = meshgen.icosphere_sh(level=6, seed=11)
+    v = v + np.float32(-0.5)
+    dv = torch.empty(v.size, dtype=torch.float32, device=gpu)
+    df = torch.empty(f.size, dtype=torch.int32, device=gpu)
+    for rep in range(3):
+        dv.fill_(7.0)
+        df.fill_(0)
+        dv.copy_(torch.from_numpy(v.reshape(-1)).to(gpu))  # written on the current stream
+        df.copy_(torch.from_numpy(f.reshape(-1)).to(gpu))
+        box = V.mesh_bbox_device(dv, stream=side)
+        np.testing.assert_array_equal(box.min, v.min(0))
+        np.testing.assert_array_equal(box.max, v.max(0))
+        tris = V.pack_triangles(dv, df, stream=side)
+        np.testing.assert_array_equal(tris.cpu().numpy(), meshgen.to_aos(v, f).reshape(-1))
+        info = V.voxinfo_for_mesh(v, 128, f.shape[0])
+        table = V.voxelize_surface(info, tris, stream=side)
+        side.synchronize()
+        exp, _ = oracle.surface(oracle.info_from_bytes(info.as_bytes()), (v, f))
+        assert (table.cpu().numpy().view(np.uint32) == exp).all()
+        assert V.count_voxels(table, stream=side) == int(np.unpackbits(exp.view(np.uint8)).sum())
+        mor = V.convert_layout(table, 128, True, stream=side)
+        side.synchronize()
+        exp_m, _ = oracle.surface(oracle.info_from_bytes(info.as_bytes()), (v, f), morton=True)
+        assert (mor.cpu().numpy().view(np.uint32) == exp_m).all()
