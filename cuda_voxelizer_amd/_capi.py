@@ -135,6 +135,17 @@ def load() -> ctypes.CDLL:
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
             "(hipcc, gfx950). There is no CPU fallback.")
+    # One HIP runtime per process.  The torch wheel carries its own
+    # libamdhip64 / libhsa-runtime64 under the same SONAMEs as /opt/rocm's but
+    # asks for them by unversioned file name, so if this library (which needs
+    # the versioned SONAMEs) were mapped first, importing torch would map a
+    # second HIP and HSA runtime, and the one this library calls then finds
+    # no device.  Importing torch first lets the dynamic linker bind this
+    # library to the runtime torch already holds.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
