@@ -96,6 +96,8 @@ SIGNATURES = {
     "vox_bbox_cube": (ctypes.c_int, [_F3, _F3, _F3, _F3]),
     "vox_surface": (ctypes.c_int, [ctypes.POINTER(vox_info), _P, _P, ctypes.c_int, ctypes.c_uint, _P]),
     "vox_solid": (ctypes.c_int, [ctypes.POINTER(vox_info), _P, _P, ctypes.c_int, ctypes.c_uint, _P, _P]),
+    "vox_solid_noalloc": (ctypes.c_int, [ctypes.POINTER(vox_info), _P, _P, _P, _P]),
+    "vox_noalloc_scratch_bytes": (ctypes.c_size_t, []),
     "vox_query_tile_plan": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]),
     "vox_batch_create": (ctypes.c_int, [ctypes.POINTER(vox_batch_item), ctypes.c_int, ctypes.c_uint,
                                         ctypes.POINTER(ctypes.c_void_p)]),

@@ -187,6 +187,9 @@ vox_status vox_init(int device) {
     if (device < 0 || device >= count) return fail(VOX_ERR_INVALID, "device index out of range");
     e = hipSetDevice(device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    // the device address of vox_solid_noalloc's static scratch on this GPU
+    e = static_scratch_init(device, nullptr);
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
     return VOX_OK;
 }
 
@@ -256,6 +259,23 @@ vox_status vox_solid(const vox_info* info, const float* d_tris, uint32_t* d_tabl
                      uint64_t* d_offrow, void* stream) {
     return run(info, d_tris, d_table, layout, flags, d_offrow, stream, true);
 }
+
+vox_status vox_solid_noalloc(const vox_info* info, const float* d_tris, uint32_t* d_table, uint64_t* d_offrow,
+                             void* stream) {
+    vox_status s = validate(info, d_tris, d_table, VOX_LAYOUT_LINEAR);
+    if (s != VOX_OK) return s;
+    const uint32_t G = info->gridsize[0];
+    if (!marker_supported(G, VOX_LAYOUT_LINEAR, false))
+        return fail(VOX_ERR_UNSUPPORTED,
+                    "vox_solid_noalloc: the grid edge must be a multiple of 32 between 32 and 16384 "
+                    "(its fixed scratch holds only the spill counters and records)");
+    const hipError_t e = launch_solid_marker_static(*info, d_tris, d_table, (unsigned long long*)d_offrow,
+                                                    (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "vox_solid_noalloc launch");
+    return VOX_OK;
+}
+
+size_t vox_noalloc_scratch_bytes(void) { return static_scratch_bytes(); }
 
 }  // extern "C"
 

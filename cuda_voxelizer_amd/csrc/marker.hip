@@ -738,6 +738,9 @@ bool marker_supported(uint32_t G, int layout, bool accumulate) {
     return true;
 }
 
+hipError_t launch_marker_in_place(const vox_info& info, const float* tris, uint32_t* table,
+                                  unsigned long long* offrow, const MarkerWs& ws, hipStream_t stream);
+
 hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t* table, int layout,
                                bool accumulate, unsigned long long* offrow, hipStream_t stream) {
     const uint32_t G = info.gridsize[0];
@@ -777,13 +780,14 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
             b.scratch_dirty = true;  // until this call's brick pass is queued
         }
     }
+    if (in_place) return launch_marker_in_place(info, tris, table, offrow, ws, stream);
     if (need_fill) {
         e = launch_fill_zero(info, markers, stream);
         if (e != hipSuccess) return e;
     }
     const bool linear = layout == VOX_LAYOUT_LINEAR;
     const ItemMap none{nullptr, nullptr, 1, 0};
-    ItemDev one{};  // markers: into `markers`
+    ItemDev one{};  // markers: into the scratch table
     one.info = info;
     one.tris = tris;
     one.table = markers;
@@ -796,13 +800,47 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
         const uint64_t blocks = (info.n_triangles + kMBlock - 1) / kMBlock;
         solid_marker_kernel<false><<<(unsigned)blocks, kMBlock, 0, stream>>>(one, none, ws, linear ? 1 : 0);
     }
-    if (in_place && (G & 127u) == 0 && (((uintptr_t)table) & 15u) == 0) {
+    const uint32_t rz = linear ? 1u : (G <= 1024u ? 8u : 2u);
+    const uint32_t nblocks = (G / 32u) * (G / rz);
+    marker_build_kernel<true><<<nblocks, 256, 4ull * 32u * rz * (G / 32u + 1u), stream>>>(markers, table, G, rz, layout,
+                                                                             accumulate ? 1 : 0);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    {
+        std::lock_guard<std::mutex> lock(g_mws_mutex);
+        g_mws[{dev, stream}].scratch_dirty = false;  // every row of the G^3 table was read and cleared
+    }
+    if (linear) spill_kernel<<<kSpillBlocks, 256, 0, stream>>>(fin, none, none, ws);
+    return hipGetLastError();
+}
+
+// Linear layout, no accumulation: zero-fill the table, markers into it, the
+// in-place row suffix, then the spilled runs.  `ws` holds the counters (zero
+// on entry; the spill pass returns them to zero) and the spill records.
+hipError_t launch_marker_in_place(const vox_info& info, const float* tris, uint32_t* table,
+                                  unsigned long long* offrow, const MarkerWs& ws, hipStream_t stream) {
+    const uint32_t G = info.gridsize[0];
+    hipError_t e = launch_fill_zero(info, table, stream);
+    if (e != hipSuccess) return e;
+    const ItemMap none{nullptr, nullptr, 1, 0};
+    ItemDev one{};
+    one.info = info;
+    one.tris = tris;
+    one.table = table;
+    one.offrow = offrow;
+    one.layout = VOX_LAYOUT_LINEAR;
+    one.solid = 1;
+    if (info.n_triangles) {
+        const uint64_t blocks = (info.n_triangles + kMBlock - 1) / kMBlock;
+        solid_marker_kernel<false><<<(unsigned)blocks, kMBlock, 0, stream>>>(one, none, ws, 1);
+    }
+    if ((G & 127u) == 0 && (((uintptr_t)table) & 15u) == 0) {
         const uint64_t nrows = (uint64_t)G * G;
         const uint64_t per = 64 / (G >> 7);
         uint64_t blocks = (nrows / per + 3) / 4;
         if (blocks > 262144) blocks = 262144;
         suffix_rows4_kernel<<<(unsigned)blocks, 256, 0, stream>>>(table, G, nrows);
-    } else if (in_place) {
+    } else {
         const uint64_t nrows = (uint64_t)G * G;
         const uint32_t rw = G >> 5;
         const uint64_t rows_per_wave = rw <= 64 ? 64 / rw : 1;
@@ -810,20 +848,59 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
         uint64_t blocks = (waves + 3) / 4;
         if (blocks > 65536) blocks = 65536;
         suffix_rows_kernel<<<(unsigned)blocks, 256, 0, stream>>>(table, G, nrows);
-    } else {
-        const uint32_t rz = linear ? 1u : (G <= 1024u ? 8u : 2u);
-        const uint32_t nblocks = (G / 32u) * (G / rz);
-        marker_build_kernel<true><<<nblocks, 256, 4ull * 32u * rz * (G / 32u + 1u), stream>>>(markers, table, G, rz, layout,
-                                                                                 accumulate ? 1 : 0);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        std::lock_guard<std::mutex> lock(g_mws_mutex);
-        g_mws[{dev, stream}].scratch_dirty = false;  // every row of the G^3 table was read and cleared
     }
-    if (linear) {
-        spill_kernel<<<kSpillBlocks, 256, 0, stream>>>(fin, none, none, ws);
-    }
+    spill_kernel<<<kSpillBlocks, 256, 0, stream>>>(one, none, none, ws);
     return hipGetLastError();
+}
+
+// ---- the allocation-free form (vox_solid_noalloc) --------------------------------
+// Its scratch is this file-scope device array: the spill counters (256 bytes)
+// and kSpillCap spill records, 1 MiB + 256 bytes in all, whatever the input
+// (more spilled runs than records are recomputed from the triangles, which
+// needs no further memory).  Each GPU holds its own copy until the process
+// ends.  In host code `scratch` only names the symbol: its device address on
+// the current GPU comes from hipGetSymbolAddress, taken once per device.
+constexpr size_t kStaticScratchBytes = 256 + 16ull * kSpillCap;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxStaticDevices = 64;
+static std::mutex g_static_mutex;
+static void* g_static_ptr[kMaxStaticDevices];  // device address of `scratch`, per device
+
+// Zero the counter words (a kernel, so the sequence has no memset node when a
+// caller captures it into a graph).
+__global__ void __launch_bounds__(64) clear_counters_kernel(unsigned* misc, int words) {
+    for (int i = threadIdx.x; i < words; i += 64) misc[i] = 0u;
+}
+
+size_t static_scratch_bytes() { return kStaticScratchBytes; }
+
+hipError_t static_scratch_init(int device, void** out) {
+    if (device < 0 || device >= kMaxStaticDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_static_mutex);
+    if (!g_static_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));  // the current device's copy
+        if (e != hipSuccess) return e;
+        g_static_ptr[device] = p;
+    }
+    if (out) *out = g_static_ptr[device];
+    return hipSuccess;
+}
+
+hipError_t launch_solid_marker_static(const vox_info& info, const float* tris, uint32_t* table,
+                                      unsigned long long* offrow, hipStream_t stream) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    void* p = nullptr;
+    e = static_scratch_init(dev, &p);
+    if (e != hipSuccess) return e;
+    const MarkerWs ws{(unsigned*)p, (unsigned long long*)((char*)p + 256)};
+    clear_counters_kernel<<<1, 64, 0, stream>>>(ws.misc, 64);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_marker_in_place(info, tris, table, offrow, ws, stream);
 }
 
 // ---- batched form (linear layout, no accumulation: in place) -------------------

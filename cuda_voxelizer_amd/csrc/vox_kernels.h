@@ -36,6 +36,12 @@ hipError_t launch_marker_jobs(const ItemMap& m_blocks, const ItemMap& m_rows, co
                               uint32_t blocks, uint32_t row_units, void* mws, hipStream_t stream);
 hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t* table, int layout,
                                bool accumulate, unsigned long long* offrow, hipStream_t stream);
+// The allocation-free solid form: its scratch is a static device array of
+// static_scratch_bytes() bytes per GPU (address taken once per device).
+size_t static_scratch_bytes();
+hipError_t static_scratch_init(int device, void** out);
+hipError_t launch_solid_marker_static(const vox_info& info, const float* tris, uint32_t* table,
+                                      unsigned long long* offrow, hipStream_t stream);
 
 bool chunk_supported(uint32_t G);
 // Default algorithm: a mesh with few triangles for its grid has large voxel

@@ -110,6 +110,24 @@ vox_status vox_surface(const vox_info* info, const float* d_tris, uint32_t* d_ta
 vox_status vox_solid(const vox_info* info, const float* d_tris, uint32_t* d_table, int layout,
                      unsigned flags, uint64_t* d_offrow, void* stream);
 
+/* vox_solid without any device allocation.  It has no counterpart in the
+   reference: it is for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   Same table and *d_offrow as vox_solid(info, d_tris, d_table,
+   VOX_LAYOUT_LINEAR, 0, d_offrow, stream), with the marker kernels; the
+   table is fully defined by the call (no accumulation).  Limit: the grid edge
+   must be a multiple of 32 from 32 to 16384 (VOX_ERR_UNSUPPORTED otherwise).
+   Its scratch -- spill counters plus 65,536 out-of-row run records,
+   vox_noalloc_scratch_bytes() = 1 MiB + 256 bytes whatever the input (more
+   spilled runs are recomputed from the triangles) -- is a static device array
+   of the library, one copy per GPU, held until the process ends; its address
+   is taken in vox_init(device), or at the first call on a device.  Every
+   call on one GPU shares that copy: calls that overlap on one GPU (different
+   streams without ordering between them) race, so serialise them. */
+vox_status vox_solid_noalloc(const vox_info* info, const float* d_tris, uint32_t* d_table,
+                             uint64_t* d_offrow, void* stream);
+size_t vox_noalloc_scratch_bytes(void);
+
 /* ---- mesh preparation and table utilities (the steps either side of the path) --- */
 /* Component-wise min/max of n_vertices xyz vertices (trimesh2 need_bbox, used at
    main.cpp:179), written to d_out6 = {min.xyz, max.xyz} (device memory). */
