@@ -26,8 +26,9 @@ VOX_FLAG_ALGO_DIRECT = 0x2
 VOX_FLAG_ALGO_TILE = 0x4
 VOX_FLAG_ALGO_CHUNK = 0x8
 VOX_FLAG_ALGO_MARKER = 0x10
+VOX_FLAG_ALGO_SLAB = 0x20
 ALGO_FLAGS = {"auto": 0, "direct": VOX_FLAG_ALGO_DIRECT, "tile": VOX_FLAG_ALGO_TILE, "chunk": VOX_FLAG_ALGO_CHUNK,
-              "marker": VOX_FLAG_ALGO_MARKER}
+              "marker": VOX_FLAG_ALGO_MARKER, "slab": VOX_FLAG_ALGO_SLAB}
 
 _STATUS_NAMES = {0: "VOX_OK", 1: "VOX_ERR_INVALID", 2: "VOX_ERR_HIP", 3: "VOX_ERR_OOM",
                  4: "VOX_ERR_UNSUPPORTED", 5: "VOX_ERR_NO_DEVICE"}

@@ -111,6 +111,48 @@ vox_status get_workspace(hipStream_t st, const TilePlan& P, uint64_t n, void** o
     return VOX_OK;
 }
 
+// ---- per (device, stream) workspace of the slab pipeline -------------------
+// Its misc counters and per-tile counts are left zero by every call, so they
+// are zero-filled only when the workspace is (re)allocated.
+struct SlabBuffer {
+    void* ptr = nullptr;
+    SlabLayout L{};
+};
+static std::mutex g_slab_mutex;
+static std::map<std::pair<int, hipStream_t>, SlabBuffer> g_slab;
+
+vox_status get_slab_workspace(hipStream_t st, const SlabPlan& P, uint64_t n, void** out, SlabLayout* lay) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    std::lock_guard<std::mutex> lock(g_slab_mutex);
+    SlabBuffer& W = g_slab[std::make_pair(dev, st)];
+    if (W.ptr == nullptr || !slab_layout_fits(P, n, W.L)) {
+        const uint64_t tiles_cap = W.L.tiles_cap > (uint64_t)P.ntiles ? W.L.tiles_cap : (uint64_t)P.ntiles;
+        uint64_t n_cap = W.L.n_cap > n ? W.L.n_cap : n;
+        if (n_cap < 4096) n_cap = 4096;
+        const SlabLayout L = slab_layout(tiles_cap, n_cap);
+        if (!slab_layout_fits(P, n, L)) return fail(VOX_ERR_INVALID, "slab workspace layout does not fit its plan");
+        if (W.ptr) {
+            e = hipStreamSynchronize(st);  // the old buffer may still be in use on this stream
+            if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+            (void)hipFree(W.ptr);
+            W.ptr = nullptr;
+        }
+        e = hipMalloc(&W.ptr, L.total);
+        if (e != hipSuccess) {
+            W = SlabBuffer{};
+            return fail(VOX_ERR_OOM, std::string("slab workspace hipMalloc: ") + hipGetErrorString(e));
+        }
+        e = hipMemset(W.ptr, 0, L.zero_bytes);
+        if (e != hipSuccess) return hip_fail(e, "hipMemset(slab workspace)");
+        W.L = L;
+    }
+    *out = W.ptr;
+    *lay = W.L;
+    return VOX_OK;
+}
+
 // One voxelization.  Algorithms: tile (binned, tile-owned writes; needs a
 // plan for this grid/layout), chunk (surface only, LDS word merging + global
 // atomics after a fill) and direct (one triangle per lane).  Without an
@@ -119,8 +161,8 @@ vox_status run(const vox_info* info, const float* d_tris, uint32_t* d_table, int
                uint64_t* d_offrow, void* stream, bool solid) {
     vox_status s = validate(info, d_tris, d_table, layout);
     if (s != VOX_OK) return s;
-    const unsigned algo_bits =
-        flags & (VOX_FLAG_ALGO_DIRECT | VOX_FLAG_ALGO_TILE | VOX_FLAG_ALGO_CHUNK | VOX_FLAG_ALGO_MARKER);
+    const unsigned algo_bits = flags & (VOX_FLAG_ALGO_DIRECT | VOX_FLAG_ALGO_TILE | VOX_FLAG_ALGO_CHUNK |
+                                        VOX_FLAG_ALGO_MARKER | VOX_FLAG_ALGO_SLAB);
     if (algo_bits & (algo_bits - 1)) return fail(VOX_ERR_INVALID, "at most one VOX_FLAG_ALGO_* flag");
     if (solid && (flags & VOX_FLAG_ALGO_CHUNK)) return fail(VOX_ERR_INVALID, "the chunk algorithm is surface-only");
     if (!solid && (flags & VOX_FLAG_ALGO_MARKER))
@@ -131,8 +173,15 @@ vox_status run(const vox_info* info, const float* d_tris, uint32_t* d_table, int
     TilePlan P;
     const bool have_plan = make_plan((int)G, layout, solid, P);
     const bool have_marker = solid && marker_supported(G, layout, accumulate);
-    enum { DIRECT, TILE, CHUNK, MARKER } algo;
+    SlabPlan SP;
+    const int lg_ty = (int)((flags >> 8) & 15u) - 1, lg_tz = (int)((flags >> 12) & 15u) - 1;
+    const int split_lg = (int)((flags >> 16) & 31u) - 1;
+    const int wg = (int)((flags >> 21) & 3u);
+    const bool have_slab = layout == VOX_LAYOUT_LINEAR && (((uintptr_t)d_table) & 15u) == 0 &&
+                           make_slab_plan(G, SP, lg_ty, lg_tz, split_lg, wg);
+    enum { DIRECT, TILE, CHUNK, MARKER, SLAB } algo;
     if (flags & VOX_FLAG_ALGO_DIRECT) algo = DIRECT;
+    else if (flags & VOX_FLAG_ALGO_SLAB) algo = have_slab ? SLAB : (have_plan ? TILE : DIRECT);
     else if (flags & VOX_FLAG_ALGO_TILE) algo = have_plan ? TILE : DIRECT;
     else if (flags & VOX_FLAG_ALGO_CHUNK) algo = chunk_supported(G) ? CHUNK : DIRECT;
     else if (flags & VOX_FLAG_ALGO_MARKER) algo = have_marker ? MARKER : (have_plan ? TILE : DIRECT);
@@ -141,6 +190,15 @@ vox_status run(const vox_info* info, const float* d_tris, uint32_t* d_table, int
     else if (have_plan && prefer_tile_surface(info->n_triangles, G)) algo = TILE;
     else algo = chunk_supported(G) ? CHUNK : (have_plan ? TILE : DIRECT);
     hipError_t e;
+    if (algo == SLAB) {
+        void* ws = nullptr;
+        SlabLayout L;
+        s = get_slab_workspace(st, SP, info->n_triangles, &ws, &L);
+        if (s != VOX_OK) return s;
+        e = launch_slab(*info, d_tris, d_table, SP, solid, accumulate, (unsigned long long*)d_offrow, ws, L, st);
+        if (e != hipSuccess) return hip_fail(e, "slab kernels launch");
+        return VOX_OK;
+    }
     if (algo == MARKER) {
         e = launch_solid_marker(*info, d_tris, d_table, layout, accumulate, (unsigned long long*)d_offrow, st);
         if (e != hipSuccess) return hip_fail(e, "marker kernels launch");

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "../../include/cuda_voxelizer_amd/vox_capi.h"
+#include "slab.h"
 #include "tile.h"
 
 namespace voxamd {
@@ -42,6 +43,13 @@ size_t static_scratch_bytes();
 hipError_t static_scratch_init(int device, void** out);
 hipError_t launch_solid_marker_static(const vox_info& info, const float* tris, uint32_t* table,
                                       unsigned long long* offrow, hipStream_t stream);
+
+bool make_slab_plan(uint32_t G, SlabPlan& P, int lg_ty = -1, int lg_tz = -1, int split_lg = -1, int wg = 0);
+SlabLayout slab_layout(uint64_t tiles_cap, uint64_t n_cap);
+bool slab_layout_fits(const SlabPlan& P, uint64_t n, const SlabLayout& L);
+hipError_t launch_slab(const vox_info& info, const float* tris, uint32_t* table, const SlabPlan& P, bool solid,
+                       bool accumulate, unsigned long long* offrow, void* mem, const SlabLayout& L,
+                       hipStream_t stream);
 
 bool chunk_supported(uint32_t G);
 // Default algorithm: a mesh with few triangles for its grid has large voxel

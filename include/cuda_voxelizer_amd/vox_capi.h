@@ -74,6 +74,23 @@ typedef enum vox_status {
                                     in a zeroed bit table (the output itself for
                                     the linear layout), then every row becomes the
                                     suffix XOR of its markers along x */
+#define VOX_FLAG_ALGO_SLAB 0x20u  /* linear layout, G % 32 == 0, G <= 2048: triangles
+                                    binned to slabs of whole x rows; one workgroup
+                                    builds each slab in LDS (surface: OR of the
+                                    passing voxels; solid: run-end markers and the
+                                    row suffix XOR) and writes every word of it
+                                    once -- no zero-fill pass, no global atomics
+                                    except for rare wide triangles and runs that
+                                    leave their row */
+/* Slab tuning bits (with VOX_FLAG_ALGO_SLAB; a zero field is the library's
+   choice).  Every setting gives the same table. */
+#define VOX_SLAB_TY(lg) ((((unsigned)(lg) + 1u) & 15u) << 8)     /* 2^lg rows in y per tile */
+#define VOX_SLAB_TZ(lg) ((((unsigned)(lg) + 1u) & 15u) << 12)    /* 2^lg rows in z per tile */
+#define VOX_SLAB_SPLIT(lg) ((((unsigned)(lg) + 1u) & 31u) << 16) /* surface: tiles with more than
+                                    2^lg triangle entries are cut along x into 2, 4 or 8
+                                    parts, one workgroup each (lg >= 30: never) */
+#define VOX_SLAB_WG(k) ((((unsigned)(k)) & 3u) << 21) /* raster workgroups: 1 = 1024 threads,
+                                    2 = 256 threads (the default) */
 
 /* ---- setup / diagnostics ---------------------------------------------------- */
 vox_status vox_init(int device);            /* select + probe a device (initCuda) */

@@ -14,8 +14,8 @@ from tests.conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
-ALGOS = ["auto", "tile", "chunk", "direct"]
-SOLID_ALGOS = ["auto", "marker", "tile", "direct"]
+ALGOS = ["auto", "tile", "chunk", "direct", "slab"]
+SOLID_ALGOS = ["auto", "marker", "tile", "direct", "slab"]
 
 
 def _dev_tris(tris, dev):
