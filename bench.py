@@ -487,15 +487,21 @@ def run_batch(args) -> dict:
     meshes = meshgen.batch_meshes(256)
     jobs = _batch_jobs(meshes, dev)
     stream = torch.cuda.current_stream()
-    from cuda_voxelizer_amd.autotune import Tuner
+    from cuda_voxelizer_amd.autotune import Tuner, describe_tune
     t_tune = time.perf_counter()
     tuner = Tuner()
-    tuner.tune_jobs(jobs)  # per-class algorithm choice, cached on disk per device
+    tuner.tune_jobs(jobs)  # per-class algorithm and knobs, cached on disk per device
     t_tune = time.perf_counter() - t_tune
-    algo_mix = {}
+    algo_mix, knob_mix = {}, {}
     for j in jobs:
         algo_mix[j.algo] = algo_mix.get(j.algo, 0) + 1
+        k = f"{j.algo}:{describe_tune(j.algo, j.tune)}"
+        knob_mix[k] = knob_mix.get(k, 0) + 1
     batch = V.Batch(jobs, use_graph=True)
+    # the same algorithms with every knob at its built-in setting, for the knobs' gain
+    fixed_jobs = [V.BatchJob(j.info, j.tris, torch.empty_like(j.table), morton=j.morton, solid=j.solid, algo=j.algo)
+                  for j in jobs]
+    fixed = V.Batch(fixed_jobs, use_graph=True)
     steps = max(1, min(args.steps, 10))
 
     def timed(fn, k):
@@ -516,7 +522,10 @@ def run_batch(args) -> dict:
                 V.voxelize_surface(j.info, j.tris, table=j.table, stream=stream)
 
     t_batch = timed(lambda: batch.launch(stream), steps)
+    t_fixed = timed(lambda: fixed.launch(stream), steps)
     batch_tables = [j.table.clone() for j in jobs]
+    same_fixed = all(bool(torch.equal(a.table, b)) for a, b in zip(fixed_jobs, batch_tables))
+    fixed.close()
     t_plain = timed(plain, steps)
     same = all(bool((a == j.table).all()) for a, j in zip(batch_tables, jobs))
     ntri = sum(m.tris.shape[0] for m in meshes)
@@ -532,6 +541,8 @@ def run_batch(args) -> dict:
         "triangles_per_s": ntri / t_batch,
         "plain_meshes_per_s": len(meshes) / t_plain, "speedup_vs_plain": t_plain / t_batch,
         "batch_equals_plain": same, "batch_info": batch.info(), "autotune_s": t_tune, "algo_mix": algo_mix,
+        "knob_mix": knob_mix, "fixed_knobs_meshes_per_s": len(meshes) / t_fixed,
+        "gain_over_fixed_knobs": t_fixed / t_batch, "tuned_equals_fixed_knobs": same_fixed,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": alg_bytes,
                      "launch_ms": t_batch * 1e3},

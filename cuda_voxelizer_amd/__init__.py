@@ -220,17 +220,18 @@ def _info_checked(info: VoxInfo, tris, table, grid_words_needed=True):
 
 
 def voxelize_surface(info: VoxInfo, tris, morton: bool = False, table=None, accumulate: bool = False,
-                     algo: str = "auto", stream=None):
+                     algo: str = "auto", stream=None, tune: int = 0):
     """Surface voxelization (stream-ordered).  Returns the int32 table tensor.
 
     Without ``accumulate`` the table is fully defined by this call; with it
-    bits are OR-ed into the existing contents (reference semantics)."""
+    bits are OR-ed into the existing contents (reference semantics).
+    ``tune``: VOX_TUNE_* bits (``_capi.tune_*``); every setting gives the same table."""
     torch = _torch()
     with _OnStream(stream, tris.device) as on:
         if table is None:
             table = torch.empty(table_words(info.grid), dtype=torch.int32, device=tris.device)
         _info_checked(info, tris, table)
-        flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | _algo_flag(algo)
+        flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | _algo_flag(algo) | (int(tune) & _capi.TUNE_MASK)
         lib = _capi.load()
         check(lib.vox_surface(ctypes.byref(info._c), tris.data_ptr(), table.data_ptr(),
                               VOX_LAYOUT_MORTON if morton else VOX_LAYOUT_LINEAR, flags, on.ptr))
@@ -238,7 +239,7 @@ def voxelize_surface(info: VoxInfo, tris, morton: bool = False, table=None, accu
 
 
 def voxelize_solid_table(info: VoxInfo, tris, morton: bool = False, table=None, accumulate: bool = False,
-                         algo: str = "auto", stream=None, offrow=None):
+                         algo: str = "auto", stream=None, offrow=None, tune: int = 0):
     """Solid (parity) voxelization (stream-ordered).  Returns the int32 table.
 
     ``offrow``: optional int64 device tensor of one element to which the
@@ -251,7 +252,7 @@ def voxelize_solid_table(info: VoxInfo, tris, morton: bool = False, table=None, 
         if table is None:
             table = torch.empty(table_words(info.grid), dtype=torch.int32, device=tris.device)
         _info_checked(info, tris, table)
-        flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | _algo_flag(algo)
+        flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | _algo_flag(algo) | (int(tune) & _capi.TUNE_MASK)
         cl_ptr = None
         if offrow is not None:
             _check_tensor(offrow, "torch.int64", "offrow")
@@ -293,6 +294,7 @@ class BatchJob:
     accumulate: bool = False
     algo: str = "auto"
     offrow: object = None        # solid: optional int64 device tensor of 1 element
+    tune: int = 0                # VOX_TUNE_* bits (autotuner knobs; the table is the same)
 
 
 class Batch:
@@ -315,7 +317,7 @@ class Batch:
             it.d_offrow = j.offrow.data_ptr() if j.offrow is not None else None
             it.layout = VOX_LAYOUT_MORTON if j.morton else VOX_LAYOUT_LINEAR
             it.solid = 1 if j.solid else 0
-            it.flags = (VOX_FLAG_ACCUMULATE if j.accumulate else 0) | _algo_flag(j.algo)
+            it.flags = (VOX_FLAG_ACCUMULATE if j.accumulate else 0) | _algo_flag(j.algo) | (int(j.tune) & _capi.TUNE_MASK)
         handle = ctypes.c_void_p()
         check(lib.vox_batch_create(arr, len(jobs), 0 if use_graph else _capi.VOX_BATCH_NO_GRAPH,
                                    ctypes.byref(handle)))

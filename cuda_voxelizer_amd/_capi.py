@@ -30,6 +30,26 @@ VOX_FLAG_ALGO_SLAB = 0x20
 ALGO_FLAGS = {"auto": 0, "direct": VOX_FLAG_ALGO_DIRECT, "tile": VOX_FLAG_ALGO_TILE, "chunk": VOX_FLAG_ALGO_CHUNK,
               "marker": VOX_FLAG_ALGO_MARKER, "slab": VOX_FLAG_ALGO_SLAB}
 
+
+
+def tune_big(lg: int) -> int:
+    """VOX_TUNE_BIG: queue threshold 2^lg (chunk: voxels per box; marker: columns)."""
+    return ((lg + 1) & 15) << 24
+
+
+def tune_waves(w: int) -> int:
+    """VOX_TUNE_WAVES: chunk kernel at 5 (w=1) or 6 (w=2) waves per SIMD."""
+    return (w & 3) << 28
+
+
+def tune_tile(t: int) -> int:
+    """VOX_TUNE_TILE: tile algorithm at half (t=1) or twice (t=3) the built-in tile size."""
+    return (t & 3) << 30
+
+
+TUNE_MASK = 0xFF000000
+
+
 _STATUS_NAMES = {0: "VOX_OK", 1: "VOX_ERR_INVALID", 2: "VOX_ERR_HIP", 3: "VOX_ERR_OOM",
                  4: "VOX_ERR_UNSUPPORTED", 5: "VOX_ERR_NO_DEVICE"}
 

@@ -21,6 +21,14 @@ path: a batch lasts at least as long as its slowest job, so a candidate whose
 single-copy time exceeds ``latency_slack`` times the best one (plus 20 us of
 launch noise) is ruled out, and the fastest per copy among the rest wins.
 
+Once the algorithm is chosen, its knobs are tuned the same way, one axis at
+a time starting from the built-in setting (``KNOBS``): the chunk kernel's
+waves per SIMD (5 or 6, two instantiations) and its queue threshold (voxel
+boxes above 64, 256 or 1024 voxels are shared by the workgroup), the marker
+kernel's queue threshold (16, 64 or 256 columns), and the tile algorithm's
+tile size (half, built-in, twice).  Every setting gives the same table; the
+chosen VOX_TUNE_* bits are cached with the algorithm.
+
 Cache file: $VOX_AUTOTUNE_CACHE, else ~/.cache/cuda_voxelizer_amd/autotune.json.
 """
 from __future__ import annotations
@@ -32,6 +40,40 @@ import time
 
 SURFACE_ALGOS = ("chunk", "tile", "direct")
 SOLID_ALGOS = ("marker", "tile", "direct")
+
+
+def _knobs():
+    from ._capi import tune_big, tune_tile, tune_waves
+    # algorithm -> axes, each a list of (label, VOX_TUNE_* bits); 0 is the built-in setting
+    return {
+        "chunk": [[("waves=5", tune_waves(1)), ("waves=6", tune_waves(2))],
+                  [("big=64", tune_big(6)), ("big=256", 0), ("big=1024", tune_big(10))]],
+        "marker": [[("big=16", tune_big(4)), ("big=64", 0), ("big=256", tune_big(8))]],
+        "tile": [[("tile=half", tune_tile(1)), ("tile=1", 0), ("tile=twice", tune_tile(3))]],
+        "direct": [],
+    }
+
+
+def describe_tune(algo: str, tune: int) -> str:
+    """Readable form of a job's knob bits for its algorithm ("-" when it has none)."""
+    labels = []
+    for axis in _knobs().get(algo, []):
+        v = tune & _axis_mask(axis)
+        hit = [lab for lab, bits in axis if bits == v]
+        labels.append(hit[0] if hit else axis[0][0].split("=")[0] + "=built-in")
+    return ",".join(labels) if labels else "-"
+
+
+def _axis_mask(axis) -> int:
+    m = 0
+    for _, bits in axis:
+        m |= bits
+    # the field the axis's bits live in (a byte-aligned nibble or two-bit field)
+    for lo, width in ((24, 4), (28, 2), (30, 2)):
+        field = ((1 << width) - 1) << lo
+        if m & field:
+            return field
+    return m
 
 
 def default_cache_path() -> str:
@@ -48,13 +90,15 @@ def class_key(grid: int, morton: bool, solid: bool, n_triangles: int) -> str:
 
 class Tuner:
     def __init__(self, path: str | None = None, device_name: str | None = None, reps: int = 3, timer=None,
-                 fill_triangles: int = 2_000_000, max_table_bytes: int = 512 << 20, latency_slack: float = 2.0):
+                 fill_triangles: int = 2_000_000, max_table_bytes: int = 512 << 20, latency_slack: float = 2.0,
+                 tune_knobs: bool = True):
         self.path = path or default_cache_path()
+        self.tune_knobs = tune_knobs
         self.reps = reps
         self.fill_triangles = fill_triangles
         self.max_table_bytes = max_table_bytes
         self.latency_slack = latency_slack
-        self._timer = timer  # callable(job, algo) -> seconds; default times on the GPU
+        self._timer = timer  # callable(job, algo, tune) -> seconds or (single, per copy); default: the GPU
         self.device_name = device_name or self._device_name()
         self.table: dict = {}
         self._load()
@@ -95,12 +139,13 @@ class Tuner:
         table_bytes = max(1, int(job.table.numel()) * job.table.element_size())
         return max(1, min(k, self.max_table_bytes // table_bytes))
 
-    def _time_batch(self, job, algo: str, k: int) -> float:
+    def _time_batch(self, job, algo: str, k: int, tune: int = 0) -> float:
         import torch
 
         from . import Batch, BatchJob
         tables = [torch.empty_like(job.table) for _ in range(k)]
-        copies = [BatchJob(job.info, job.tris, t, morton=job.morton, solid=job.solid, algo=algo) for t in tables]
+        copies = [BatchJob(job.info, job.tris, t, morton=job.morton, solid=job.solid, algo=algo, tune=tune)
+                  for t in tables]
         batch = Batch(copies, use_graph=True)
         try:
             batch.launch()  # warm (code load, workspace)
@@ -115,10 +160,10 @@ class Tuner:
             batch.close()
         return best
 
-    def _time_gpu(self, job, algo: str) -> tuple:
+    def _time_gpu(self, job, algo: str, tune: int = 0) -> tuple:
         """(single-copy seconds, seconds per copy in a K-copy launch)."""
         k = self.copies(job)
-        return self._time_batch(job, algo, 1), self._time_batch(job, algo, k) / k
+        return self._time_batch(job, algo, 1, tune), self._time_batch(job, algo, k, tune) / k
 
     @staticmethod
     def candidates(job) -> tuple:
@@ -144,20 +189,46 @@ class Tuner:
 
     def choose(self, job) -> str:
         """Best algorithm for `job` (a BatchJob); times the candidates on a miss."""
+        return self.choose_tuned(job)[0]
+
+    def choose_tuned(self, job) -> tuple:
+        """(algorithm, VOX_TUNE_* bits) for `job`: the algorithm as `choose`
+        picks it, then its knobs one axis at a time from the built-in setting."""
         key = class_key(job.info.grid, job.morton, job.solid, job.info.n_triangles)
         hit = self.table.get(key)
-        if hit:
-            return hit["algo"]
+        if hit and ("tune" in hit or not self.tune_knobs):
+            return hit["algo"], int(hit.get("tune", 0)) if self.tune_knobs else 0
         timer = self._timer or self._time_gpu
-        times = {a: timer(job, a) for a in self.candidates(job)}
-        algo = self.pick(times)
-        self.table[key] = {"algo": algo, "seconds": {a: list(t) if isinstance(t, tuple) else t
-                                                     for a, t in times.items()}}
-        return algo
+        if hit:
+            algo, times = hit["algo"], None
+        else:
+            times = {a: timer(job, a, 0) for a in self.candidates(job)}
+            algo = self.pick(times)
+        entry = {"algo": algo, "seconds": hit["seconds"] if hit else
+                 {a: list(t) if isinstance(t, tuple) else t for a, t in times.items()}}
+        tune = 0
+        if self.tune_knobs:
+            knob_times = {}
+            for axis in _knobs().get(algo, []):
+                mask = _axis_mask(axis)
+                opts = {}
+                for label, bits in axis:
+                    t = (tune & ~mask) | bits
+                    opts[t] = timer(job, algo, t)
+                    knob_times[label] = list(opts[t]) if isinstance(opts[t], tuple) else opts[t]
+                tune = self.pick(opts)
+            entry["tune"] = tune
+            entry["knobs"] = describe_tune(algo, tune)
+            entry["knob_seconds"] = knob_times
+        self.table[key] = entry
+        return algo, tune
 
-    def tune_jobs(self, jobs) -> list:
-        """Set every job's algorithm from the tuned table (tuning misses first)."""
+    def tune_jobs(self, jobs, knobs: bool = True) -> list:
+        """Set every job's algorithm (and, with `knobs`, its VOX_TUNE_* bits)
+        from the tuned table, tuning misses first."""
         for j in jobs:
-            j.algo = self.choose(j)
+            j.algo, tune = self.choose_tuned(j)
+            if hasattr(j, "tune"):
+                j.tune = tune if knobs else 0
         self.save()
         return jobs

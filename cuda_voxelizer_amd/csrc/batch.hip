@@ -42,7 +42,7 @@ struct vox_batch {
     void* ws = nullptr;
     void* mws = nullptr;  // marker class: spill records and counters
     voxamd::WorkspaceLayout L{};
-    voxamd::JobClass fill, chunk, marker, dsurf, dsolid, tsurf, tsolid;
+    voxamd::JobClass fill, chunk, chunkw, marker, dsurf, dsolid, tsurf, tsolid;  // chunkw: 6 waves per SIMD
     int count = 0;
     uint64_t total_triangles = 0;
 };
@@ -60,7 +60,10 @@ static hipError_t launch_all(vox_batch* b, hipStream_t s) {
     hipError_t e = hipSuccess;
     if (b->fill.total_a && (e = launch_fill_jobs(dummy, b->fill.m_a, b->fill.total_a, s)) != hipSuccess) return e;
     if (b->chunk.total_a &&
-        (e = launch_chunk_jobs(dummy, b->chunk.m_a, b->chunk.total_a, b->chunk.loop, s)) != hipSuccess)
+        (e = launch_chunk_jobs(dummy, b->chunk.m_a, b->chunk.total_a, b->chunk.loop, s, 5)) != hipSuccess)
+        return e;
+    if (b->chunkw.total_a &&
+        (e = launch_chunk_jobs(dummy, b->chunkw.m_a, b->chunkw.total_a, b->chunkw.loop, s, 6)) != hipSuccess)
         return e;
     if (!b->marker.items.empty() &&
         (e = launch_marker_jobs(b->marker.m_a, b->marker.m_b, b->marker.m_c, b->marker.total_a, b->marker.total_b,
@@ -101,7 +104,7 @@ struct BatchPlan {
 };
 
 static void reset_classes(vox_batch* b) {
-    JobClass* classes[] = {&b->fill, &b->chunk, &b->marker, &b->dsurf, &b->dsolid, &b->tsurf, &b->tsolid};
+    JobClass* classes[] = {&b->fill, &b->chunk, &b->chunkw, &b->marker, &b->dsurf, &b->dsolid, &b->tsurf, &b->tsolid};
     for (JobClass* c : classes) *c = JobClass{};
     b->count = 0;
     b->total_triangles = 0;
@@ -124,7 +127,8 @@ static vox_status plan_batch(const vox_batch_item* items, int count, vox_batch* 
         const uint32_t G = it.info.gridsize[0];
         const bool accumulate = (it.flags & VOX_FLAG_ACCUMULATE) != 0;
         TilePlan P;
-        const bool have_plan = make_plan((int)G, it.layout, solid, P);
+        const bool have_plan =
+            make_plan((int)G, it.layout, solid, P, tune_tile_scale(it.flags)) || make_plan((int)G, it.layout, solid, P);
         // the marker class tags spill records with the job index (below 2^22)
         const bool have_marker = solid && marker_batch_supported(G, it.layout, accumulate) && i < (1 << 22);
         enum { DIRECT, TILE, CHUNK, MARKER } a;
@@ -144,6 +148,7 @@ static vox_status plan_batch(const vox_batch_item* items, int count, vox_batch* 
         d.layout = it.layout;
         d.solid = solid ? 1 : 0;
         d.accumulate = accumulate ? 1 : 0;
+        d.big = tune_big(it.flags);
         const uint64_t n = it.info.n_triangles;
         b->total_triangles += n;
         if (a != TILE && !accumulate) {
@@ -160,9 +165,11 @@ static vox_status plan_batch(const vox_batch_item* items, int count, vox_batch* 
         } else if (a == CHUNK) {
             d.span = chunk_span(G, n);
             d.parts = chunk_parts(G, n);
-            if (d.span > 1u && d.parts <= 1u) b->chunk.loop = true;
-            b->chunk.items.push_back(d);
-            add_prefix(b->chunk.base_a, b->chunk.total_a, chunk_units(n, d.span, d.parts));
+            // jobs that loop over chunks run in the 4-wave form: they stay in the 5-wave class
+            JobClass& c = (tune_waves(it.flags) == 6 && !(d.span > 1u && d.parts <= 1u)) ? b->chunkw : b->chunk;
+            if (d.span > 1u && d.parts <= 1u) c.loop = true;
+            c.items.push_back(d);
+            add_prefix(c.base_a, c.total_a, chunk_units(n, d.span, d.parts));
         } else if (a == DIRECT) {
             JobClass& c = solid ? b->dsolid : b->dsurf;
             c.items.push_back(d);
@@ -184,7 +191,7 @@ static vox_status plan_batch(const vox_batch_item* items, int count, vox_batch* 
         }
     }
     // one device allocation: per class the job array and its prefix arrays
-    JobClass* classes[] = {&b->fill, &b->chunk, &b->marker, &b->dsurf, &b->dsolid, &b->tsurf, &b->tsolid};
+    JobClass* classes[] = {&b->fill, &b->chunk, &b->chunkw, &b->marker, &b->dsurf, &b->dsolid, &b->tsurf, &b->tsolid};
     size_t total = 0;
     auto reserve = [&](size_t bytes) {
         const size_t off = total;
@@ -217,7 +224,7 @@ static vox_status plan_batch(const vox_batch_item* items, int count, vox_batch* 
 // Point every class's job maps at a device copy of the packed image.
 static void bind_maps(vox_batch* b, void* dev) {
     char* dm = (char*)dev;
-    JobClass* classes[] = {&b->fill, &b->chunk, &b->marker, &b->dsurf, &b->dsolid, &b->tsurf, &b->tsolid};
+    JobClass* classes[] = {&b->fill, &b->chunk, &b->chunkw, &b->marker, &b->dsurf, &b->dsolid, &b->tsurf, &b->tsolid};
     for (JobClass* c : classes) {
         const ItemDev* it = (const ItemDev*)(dm + c->off_items);
         const int ni = (int)c->items.size();
@@ -308,7 +315,7 @@ vox_status vox_batch_info(const vox_batch* b, int64_t out[8]) {
     if (!b || !out) return fail(VOX_ERR_INVALID, "NULL argument");
     out[0] = b->count;
     out[1] = (int64_t)b->total_triangles;
-    out[2] = (int64_t)b->chunk.items.size();
+    out[2] = (int64_t)(b->chunk.items.size() + b->chunkw.items.size());
     out[3] = (int64_t)b->tsurf.items.size();
     out[4] = (int64_t)b->tsolid.items.size();
     out[5] = (int64_t)(b->dsurf.items.size() + b->dsolid.items.size());

@@ -171,7 +171,7 @@ vox_status run(const vox_info* info, const float* d_tris, uint32_t* d_table, int
     const uint32_t G = info->gridsize[0];
     const bool accumulate = (flags & VOX_FLAG_ACCUMULATE) != 0;
     TilePlan P;
-    const bool have_plan = make_plan((int)G, layout, solid, P);
+    const bool have_plan = make_plan((int)G, layout, solid, P, tune_tile_scale(flags)) || make_plan((int)G, layout, solid, P);
     const bool have_marker = solid && marker_supported(G, layout, accumulate);
     SlabPlan SP;
     const int lg_ty = (int)((flags >> 8) & 15u) - 1, lg_tz = (int)((flags >> 12) & 15u) - 1;
@@ -200,7 +200,8 @@ vox_status run(const vox_info* info, const float* d_tris, uint32_t* d_table, int
         return VOX_OK;
     }
     if (algo == MARKER) {
-        e = launch_solid_marker(*info, d_tris, d_table, layout, accumulate, (unsigned long long*)d_offrow, st);
+        e = launch_solid_marker(*info, d_tris, d_table, layout, accumulate, (unsigned long long*)d_offrow, st,
+                                tune_big(flags));
         if (e != hipSuccess) return hip_fail(e, "marker kernels launch");
         return VOX_OK;
     }
@@ -219,7 +220,7 @@ vox_status run(const vox_info* info, const float* d_tris, uint32_t* d_table, int
         if (e != hipSuccess) return hip_fail(e, "table zero-fill");
     }
     if (algo == CHUNK)
-        e = launch_surface_chunk(*info, d_tris, d_table, layout, st);
+        e = launch_surface_chunk(*info, d_tris, d_table, layout, st, tune_big(flags), tune_waves(flags));
     else if (solid)
         e = launch_solid_direct(*info, d_tris, d_table, layout, (unsigned long long*)d_offrow, st);
     else

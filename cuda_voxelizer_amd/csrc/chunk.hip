@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5)) surface_chu
             surface_box(v, info, T.lo, T.hi);
             const int nx = T.hi[0] - T.lo[0] + 1;
             const int64_t vol = (int64_t)nx * (T.hi[1] - T.lo[1] + 1) * (T.hi[2] - T.lo[2] + 1);
-            big = vol > kBigVox || nx > 32;
+            big = vol > (it.big ? (int64_t)it.big : (int64_t)kBigVox) || nx > 32;
         }
         const unsigned long long bal = __ballot(big);
         const int wave = (int)(threadIdx.x >> 6);
@@ -417,15 +417,22 @@ hipError_t launch_fill_jobs(const ItemDev& one, const ItemMap& M, uint32_t units
     return hipGetLastError();
 }
 
-hipError_t launch_chunk_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, bool loop, hipStream_t stream) {
+// waves: 5 or 6 waves per SIMD for the one-chunk form (0: 6 for the linear
+// layout, 5 for Morton -- the measured better choice for each); the chunk
+// loop always runs at 4.
+hipError_t launch_chunk_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, bool loop, hipStream_t stream,
+                             int waves) {
     if (units == 0) return hipSuccess;
+    const bool wide = waves == 6 || (waves == 0 && !M.items && one.layout == VOX_LAYOUT_LINEAR);
     if (M.items && loop)
         surface_chunk_kernel<true, true, false><<<units, kChunk, 0, stream>>>(one, M);
+    else if (M.items && wide)
+        surface_chunk_kernel<true, false, true><<<units, kChunk, 0, stream>>>(one, M);
     else if (M.items)
         surface_chunk_kernel<true, false, false><<<units, kChunk, 0, stream>>>(one, M);
     else if (loop)
         surface_chunk_kernel<false, true, false><<<units, kChunk, 0, stream>>>(one, M);
-    else if (one.layout == VOX_LAYOUT_LINEAR)
+    else if (wide)
         surface_chunk_kernel<false, false, true><<<units, kChunk, 0, stream>>>(one, M);
     else
         surface_chunk_kernel<false, false, false><<<units, kChunk, 0, stream>>>(one, M);
@@ -449,12 +456,13 @@ hipError_t launch_fill_zero(const vox_info& info, uint32_t* table, hipStream_t s
 }
 
 hipError_t launch_surface_chunk(const vox_info& info, const float* tris, uint32_t* table, int layout,
-                                hipStream_t stream) {
+                                hipStream_t stream, uint32_t big, int waves) {
     if (info.n_triangles == 0) return hipSuccess;
     const ItemMap none{nullptr, nullptr, 1, 0};
-    const ItemDev one = single_job(info, tris, table, layout);
+    ItemDev one = single_job(info, tris, table, layout);
+    one.big = big;
     return launch_chunk_jobs(one, none, chunk_units(info.n_triangles, one.span, one.parts),
-                             one.span > 1u && one.parts <= 1u, stream);
+                             one.span > 1u && one.parts <= 1u, stream, waves);
 }
 
 }  // namespace voxamd

@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(kMBlock) solid_marker_kernel(ItemDev one, Item
             const int zhi = S.zmax >= (float)(G - 1) ? G - 1 : (S.zmax >= 0.0f ? (int)S.zmax : -1);
             const int64_t cols = (yhi >= ylo && zhi >= zlo) ? (int64_t)(yhi - ylo + 1) * (zhi - zlo + 1) : 0;
             bool queued = false;
-            if (cols > kMBigCols) {
+            if (cols > (it.big ? (int64_t)it.big : (int64_t)kMBigCols)) {
                 const unsigned q = atomicAdd(&qn, 1u);
                 if (q < (unsigned)kMQueue) {
                     queue[q] = threadIdx.x;
@@ -739,10 +739,10 @@ bool marker_supported(uint32_t G, int layout, bool accumulate) {
 }
 
 hipError_t launch_marker_in_place(const vox_info& info, const float* tris, uint32_t* table,
-                                  unsigned long long* offrow, const MarkerWs& ws, hipStream_t stream);
+                                  unsigned long long* offrow, const MarkerWs& ws, hipStream_t stream, uint32_t big);
 
 hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t* table, int layout,
-                               bool accumulate, unsigned long long* offrow, hipStream_t stream) {
+                               bool accumulate, unsigned long long* offrow, hipStream_t stream, uint32_t big) {
     const uint32_t G = info.gridsize[0];
     const bool in_place = layout == VOX_LAYOUT_LINEAR && !accumulate;
     const uint64_t table_bytes = (uint64_t)G * G * G / 8;
@@ -780,7 +780,7 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
             b.scratch_dirty = true;  // until this call's brick pass is queued
         }
     }
-    if (in_place) return launch_marker_in_place(info, tris, table, offrow, ws, stream);
+    if (in_place) return launch_marker_in_place(info, tris, table, offrow, ws, stream, big);
     if (need_fill) {
         e = launch_fill_zero(info, markers, stream);
         if (e != hipSuccess) return e;
@@ -788,6 +788,7 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
     const bool linear = layout == VOX_LAYOUT_LINEAR;
     const ItemMap none{nullptr, nullptr, 1, 0};
     ItemDev one{};  // markers: into the scratch table
+    one.big = big;
     one.info = info;
     one.tris = tris;
     one.table = markers;
@@ -818,12 +819,13 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
 // in-place row suffix, then the spilled runs.  `ws` holds the counters (zero
 // on entry; the spill pass returns them to zero) and the spill records.
 hipError_t launch_marker_in_place(const vox_info& info, const float* tris, uint32_t* table,
-                                  unsigned long long* offrow, const MarkerWs& ws, hipStream_t stream) {
+                                  unsigned long long* offrow, const MarkerWs& ws, hipStream_t stream, uint32_t big) {
     const uint32_t G = info.gridsize[0];
     hipError_t e = launch_fill_zero(info, table, stream);
     if (e != hipSuccess) return e;
     const ItemMap none{nullptr, nullptr, 1, 0};
     ItemDev one{};
+    one.big = big;
     one.info = info;
     one.tris = tris;
     one.table = table;
@@ -900,7 +902,7 @@ hipError_t launch_solid_marker_static(const vox_info& info, const float* tris, u
     clear_counters_kernel<<<1, 64, 0, stream>>>(ws.misc, 64);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return launch_marker_in_place(info, tris, table, offrow, ws, stream);
+    return launch_marker_in_place(info, tris, table, offrow, ws, stream, 0);
 }
 
 // ---- batched form (linear layout, no accumulation: in place) -------------------

@@ -44,6 +44,9 @@ struct ItemDev {
     uint64_t tri_base;   // first global triangle id of this job
     uint32_t span;       // chunk algorithm: chunks per workgroup (0 or 1: one)
     uint32_t parts;      // chunk algorithm: workgroups per chunk (0 or 1: one)
+    uint32_t big;        // chunk: voxel boxes above this are shared by the workgroup; marker: columns
+                         // above this (0: the built-in threshold)
+    uint32_t pad2;
 };
 
 // Work mapping of one launch: a single job passed by value (items == nullptr)

@@ -731,7 +731,7 @@ static int ilog2(int v) {
     return r;
 }
 
-bool make_plan(int G, int layout, bool solid, TilePlan& P) {
+bool make_plan(int G, int layout, bool solid, TilePlan& P, int scale) {
     P = TilePlan{};
     P.G = G;
     P.max_tiles_per_tri = 8;
@@ -745,6 +745,9 @@ bool make_plan(int G, int layout, bool solid, TilePlan& P) {
         if (bits < lo) bits = lo;
         if (bits > hi) bits = hi;
         if (bits < cells / 65536) bits = cells / 65536;  // at most 65536 tiles
+        // tuned size: half or twice the default tile (within the limits below)
+        if (scale < 0 && bits > lo) bits /= 2;
+        if (scale > 0 && bits * 2 <= hi && bits * 2 <= 524288) bits *= 2;
         int rows = (int)(bits / (uint64_t)G);
         if (rows < 1) rows = 1;
         rows = 1 << ilog2(rows);
@@ -765,7 +768,9 @@ bool make_plan(int G, int layout, bool solid, TilePlan& P) {
         if ((G & (G - 1)) != 0) return false;
         if (!solid) {
             if (G < 4) return false;
-            const int T = G <= 64 ? (G < 8 ? G : 8) : (G <= 256 ? 16 : 32);
+            int T = G <= 64 ? (G < 8 ? G : 8) : (G <= 256 ? 16 : 32);
+            if (scale < 0 && T >= 8) T /= 2;
+            if (scale > 0 && T * 2 <= G && T * 2 <= 32) T *= 2;
             P.kind = TILE_MORTON_CUBE;
             P.tx = P.ty = P.tz = T;
             P.ntx = P.nty = P.ntz = G / T;
@@ -774,7 +779,9 @@ bool make_plan(int G, int layout, bool solid, TilePlan& P) {
             P.row_words = 0;
         } else {
             if (G < 32 || G > 2048) return false;  // rows narrower than a word: direct path
-            const int T = G >= 512 ? 16 : 8;
+            int T = G >= 512 ? 16 : 8;
+            if (scale < 0 && T >= 8) T /= 2;
+            if (scale > 0 && T * 2 <= 16 && T * 2 <= G) T *= 2;
             P.kind = TILE_MORTON_ROWS;
             P.tx = G;
             P.ty = P.tz = T;

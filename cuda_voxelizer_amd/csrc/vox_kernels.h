@@ -17,7 +17,7 @@ hipError_t launch_solid_direct(const vox_info& info, const float* tris, uint32_t
 
 uint32_t direct_units(uint64_t n_triangles);
 hipError_t launch_direct_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, bool solid, hipStream_t stream);
-bool make_plan(int G, int layout, bool solid, TilePlan& P);
+bool make_plan(int G, int layout, bool solid, TilePlan& P, int scale = 0);
 WorkspaceLayout workspace_layout(uint64_t tiles_cap, uint64_t n_cap, int max_tiles_per_tri);
 hipError_t launch_tiled_jobs(const ItemDev& one, const ItemMap& M_bin, const ItemMap& M_tile, const ItemMap& M_tri,
                              uint32_t total_bin_blocks, uint32_t total_tiles, bool solid, uint32_t max_tile_words,
@@ -36,7 +36,7 @@ uint32_t marker_row_units(uint32_t G);
 hipError_t launch_marker_jobs(const ItemMap& m_blocks, const ItemMap& m_rows, const ItemMap& m_tris,
                               uint32_t blocks, uint32_t row_units, void* mws, hipStream_t stream);
 hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t* table, int layout,
-                               bool accumulate, unsigned long long* offrow, hipStream_t stream);
+                               bool accumulate, unsigned long long* offrow, hipStream_t stream, uint32_t big = 0);
 // The allocation-free solid form: its scratch is a static device array of
 // static_scratch_bytes() bytes per GPU (address taken once per device).
 size_t static_scratch_bytes();
@@ -69,8 +69,22 @@ uint32_t chunk_span(uint32_t G, uint64_t n_triangles);
 uint32_t chunk_parts(uint32_t G, uint64_t n_triangles);
 uint32_t chunk_units(uint64_t n_triangles, uint32_t span, uint32_t parts);
 hipError_t launch_fill_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, hipStream_t stream);
-hipError_t launch_chunk_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, bool loop, hipStream_t stream);
+hipError_t launch_chunk_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, bool loop, hipStream_t stream,
+                             int waves = 0);
 hipError_t launch_surface_chunk(const vox_info& info, const float* tris, uint32_t* table, int layout,
-                                hipStream_t stream);
+                                hipStream_t stream, uint32_t big = 0, int waves = 0);
+// Tuning bits of the flags (VOX_TUNE_*): queue threshold, chunk waves per SIMD, tile size.
+inline uint32_t tune_big(unsigned flags) {
+    const unsigned lg = (flags >> 24) & 15u;
+    return lg ? (1u << (lg - 1u)) : 0u;
+}
+inline int tune_waves(unsigned flags) {
+    const unsigned w = (flags >> 28) & 3u;
+    return w == 1u ? 5 : (w == 2u ? 6 : 0);
+}
+inline int tune_tile_scale(unsigned flags) {
+    const unsigned t = (flags >> 30) & 3u;
+    return t == 1u ? -1 : (t == 3u ? 1 : 0);
+}
 
 }  // namespace voxamd

@@ -89,6 +89,15 @@ typedef enum vox_status {
 #define VOX_SLAB_SPLIT(lg) ((((unsigned)(lg) + 1u) & 31u) << 16) /* surface: tiles with more than
                                     2^lg triangle entries are cut along x into 2, 4 or 8
                                     parts, one workgroup each (lg >= 30: never) */
+/* Tuning bits for the default algorithms (autotuner knobs; a zero field is the
+   built-in choice; every setting gives the same table):
+   VOX_TUNE_BIG(lg)    chunk: voxel boxes above 2^lg voxels are shared by the workgroup
+                       (built-in 256); marker: triangles above 2^lg columns (built-in 64)
+   VOX_TUNE_WAVES(w)   chunk: 1 = 5 waves per SIMD, 2 = 6 (built-in: 6 linear, 5 Morton)
+   VOX_TUNE_TILE(t)    tile: 1 = half the built-in tile, 3 = twice (within the LDS limit) */
+#define VOX_TUNE_BIG(lg) ((((unsigned)(lg) + 1u) & 15u) << 24)
+#define VOX_TUNE_WAVES(w) ((((unsigned)(w)) & 3u) << 28)
+#define VOX_TUNE_TILE(t) ((((unsigned)(t)) & 3u) << 30)
 #define VOX_SLAB_WG(k) ((((unsigned)(k)) & 3u) << 21) /* raster workgroups: 1 = 1024 threads,
                                     2 = 256 threads (the default) */
 

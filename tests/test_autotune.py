@@ -11,7 +11,7 @@ def test_choose_times_once_per_class_and_persists(tmp_path):
     from cuda_voxelizer_amd.autotune import Tuner, class_key
     calls = []
 
-    def timer(job, algo):
+    def timer(job, algo, tune=0):
         calls.append((job.info.grid, algo))
         return {"chunk": 3.0, "marker": 4.0, "tile": 1.0, "direct": 2.0}[algo]
 
@@ -35,7 +35,7 @@ def test_choose_times_once_per_class_and_persists(tmp_path):
 def test_solid_jobs_never_get_chunk(tmp_path):
     from cuda_voxelizer_amd.autotune import Tuner
     t = Tuner(str(tmp_path / "t.json"), device_name="fake",
-              timer=lambda job, algo: {"chunk": 0.1, "marker": 3.0, "tile": 1.0, "direct": 2.0}[algo])
+              timer=lambda job, algo, tune=0: {"chunk": 0.1, "marker": 3.0, "tile": 1.0, "direct": 2.0}[algo])
     j = _job(32, True, 100)
     assert t.choose(j) == "tile"
 
@@ -43,7 +43,7 @@ def test_solid_jobs_never_get_chunk(tmp_path):
 def test_copies_fill_the_device_within_the_table_budget(tmp_path):
     import numpy as np
     from cuda_voxelizer_amd.autotune import Tuner
-    t = Tuner(str(tmp_path / "t.json"), device_name="fake", timer=lambda job, algo: 1.0,
+    t = Tuner(str(tmp_path / "t.json"), device_name="fake", timer=lambda job, algo, tune=0: 1.0,
               fill_triangles=2_000_000, max_table_bytes=64 << 20)
 
     def job(grid, n):
@@ -61,7 +61,7 @@ def test_copies_fill_the_device_within_the_table_budget(tmp_path):
 
 def test_pick_rules_out_a_long_critical_path(tmp_path):
     from cuda_voxelizer_amd.autotune import Tuner
-    t = Tuner(str(tmp_path / "t.json"), device_name="fake", timer=lambda job, algo: 1.0)
+    t = Tuner(str(tmp_path / "t.json"), device_name="fake", timer=lambda job, algo, tune=0: 1.0)
     # direct: cheapest per copy, but one copy alone runs 1.3 ms (a straggler thread) -> ruled out
     times = {"tile": (60e-6, 30e-6), "direct": (1.3e-3, 20e-6)}
     assert t.pick(times) == "tile"
@@ -73,8 +73,47 @@ def test_pick_rules_out_a_long_critical_path(tmp_path):
 
 def test_direct_is_a_candidate_for_fine_meshes_only(tmp_path):
     from cuda_voxelizer_amd.autotune import Tuner
-    t = Tuner(str(tmp_path / "t.json"), device_name="fake", timer=lambda job, algo: 1.0)
+    t = Tuner(str(tmp_path / "t.json"), device_name="fake", timer=lambda job, algo, tune=0: 1.0)
     assert "direct" not in t.candidates(_job(256, False, 1000))      # 1000 < 256^2/4
     assert "direct" in t.candidates(_job(256, False, 20000))
     assert "direct" not in t.candidates(_job(128, True, 4095))
     assert set(t.candidates(_job(32, True, 256))) == {"marker", "tile", "direct"}
+
+
+def test_knobs_tuned_one_axis_at_a_time_and_cached(tmp_path):
+    """After the algorithm, each knob axis is timed from the built-in setting
+    and the best bits are kept; the cache returns them without timing."""
+    from cuda_voxelizer_amd import _capi
+    from cuda_voxelizer_amd.autotune import Tuner, describe_tune
+    seen = []
+
+    def timer(job, algo, tune=0):
+        seen.append((algo, tune))
+        if algo != "chunk":
+            return 5.0
+        t = 2.0
+        if tune & _capi.tune_waves(3) == _capi.tune_waves(2):
+            t -= 0.5   # six waves per SIMD wins
+        if tune & (15 << 24) == _capi.tune_big(10):
+            t -= 0.25  # the larger queue threshold wins
+        return t
+
+    path = str(tmp_path / "k.json")
+    job = _job(256, False, 200_000)
+    job.tune = 0
+    t = Tuner(path, device_name="fake", timer=timer)
+    t.tune_jobs([job])
+    assert job.algo == "chunk"
+    assert job.tune == _capi.tune_waves(2) | _capi.tune_big(10)
+    assert describe_tune("chunk", job.tune) == "waves=6,big=1024"
+    # the big axis was timed with the waves axis already at its best
+    assert ("chunk", _capi.tune_waves(2) | _capi.tune_big(6)) in seen
+    n = len(seen)
+    job2 = _job(256, False, 200_000)
+    job2.tune = 0
+    Tuner(path, device_name="fake", timer=timer).tune_jobs([job2])
+    assert len(seen) == n and job2.tune == job.tune
+    job3 = _job(256, False, 200_000)
+    job3.tune = 7
+    Tuner(path, device_name="fake", timer=timer).tune_jobs([job3], knobs=False)
+    assert job3.algo == "chunk" and job3.tune == 0  # the fixed defaults, for comparison

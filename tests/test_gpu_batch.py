@@ -190,3 +190,44 @@ def test_c4_batch_256_meshes_one_graph_launch(oracle, gpu, tmp_path):
     b.launch()
     _check(oracle, jobs, host)
     b.close()
+
+
+def test_tuning_knobs_same_tables(oracle, gpu):
+    """Every knob setting the autotuner may pick gives the oracle's table, in a
+    batch (both chunk classes, marker and tile jobs side by side) and in plain
+    calls: chunk waves 5/6 and queue thresholds 16..4096 voxels, marker
+    thresholds 1..1024 columns, tile sizes half/twice."""
+    import torch
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import _capi, meshgen
+    soup = meshgen.triangle_soup(6000, seed=0x71, mu=-3.0, sigma=1.0)
+    torus = meshgen.torus_slivers(nu=128, nv=64, grid=128, seed=0x72)
+    v, f = meshgen.icosphere_sh(level=4, seed=0x73)
+    closed = meshgen.to_aos(v, f)
+    coarse = meshgen.to_aos(*meshgen.icosphere_jitter(level=1, seed=0x74))
+    cases = [(soup, 128, False, "chunk", _capi.tune_waves(1) | _capi.tune_big(4)),
+             (soup, 128, False, "chunk", _capi.tune_waves(2) | _capi.tune_big(12)),
+             (torus, 256, False, "chunk", _capi.tune_waves(2)),
+             (torus, 64, True, "marker", _capi.tune_big(0)),
+             (closed, 128, True, "marker", _capi.tune_big(10)),
+             (coarse, 128, True, "tile", _capi.tune_tile(1)),
+             (coarse, 256, False, "tile", _capi.tune_tile(3)),
+             (soup, 96, False, "tile", _capi.tune_tile(3))]
+    jobs, host = [], []
+    for tris, grid, solid, algo, tune in cases:
+        info = V.voxinfo_for_mesh(tris.reshape(-1, 3), grid, tris.shape[0])
+        d = torch.from_numpy(np.ascontiguousarray(tris).reshape(-1)).to(gpu)
+        t = torch.full((V.table_words(grid),), -1, dtype=torch.int32, device=gpu)
+        jobs.append(V.BatchJob(info, d, t, solid=solid, algo=algo, tune=tune))
+        host.append(tris)
+    b = V.Batch(jobs)
+    b.launch()
+    _check(oracle, jobs, host)
+    b.close()
+    for j, tris in zip(jobs, host):  # the same settings as plain calls
+        j.table.fill_(-1)
+        if j.solid:
+            V.voxelize_solid_table(j.info, j.tris, table=j.table, algo=j.algo, tune=j.tune)
+        else:
+            V.voxelize_surface(j.info, j.tris, table=j.table, algo=j.algo, tune=j.tune)
+    _check(oracle, jobs, host)
