@@ -62,6 +62,15 @@ static_assert(offsetof(voxinfo, unit) == offsetof(vox_info, unit), "unit offset"
 // createMeshBBCube for float3 boxes (util.h:80-110), evaluated by the library.
 AABox<float3> createMeshBBCube(AABox<float3> box);
 
+// The reference's template form, called as createMeshBBCube<float3>(box)
+// (util.h:80, main.cpp:184); only float3 boxes are voxelized.
+template <typename T>
+AABox<T> createMeshBBCube(AABox<T> box);
+template <>
+inline AABox<float3> createMeshBBCube<float3>(AABox<float3> box) {
+    return createMeshBBCube(box);  // the non-template overload above
+}
+
 // Surface voxelization: reference v0.6 form and the 5-argument form.
 void voxelize(const voxinfo& v, float* triangle_data, unsigned int* vtable, bool morton_code);
 void voxelize(const voxinfo& v, float* triangle_data, unsigned int* vtable, bool useThrustPath,
