@@ -9,9 +9,11 @@ and the CUDA runtime headers, neither present), so the pins are:
   * closed forms (bit interleave, exact power-of-two division, MSB-first layout);
   * independent float64 brute force (tests/bruteforce.py: 13-axis SAT for the
     surface test, +x ray parity with truncation for the solid fill), away from ties;
-  * hand-derived known answers for degenerate and axis-aligned triangles.
+  * hand-derived known answers for degenerate and axis-aligned triangles
+    (the solid thresholds, where a float compare in place of the reference's
+    double one changes the answer, are pinned in test_oracle_thresholds.py).
 Each check is chosen so that a plausible slip in the oracle (swapped axis,
-dropped term, wrong sign, wrong bit order, float vs double compare) fails it.
+dropped term, wrong sign, wrong bit order) fails it.
 """
 import hashlib
 import json
@@ -249,7 +251,8 @@ def test_solid_single_triangle_closed_form(oracle):
 
 
 def test_solid_parallel_to_x_is_skipped(oracle):
-    # |n.x| < 1e-6 (a triangle containing the x direction) contributes nothing
+    # n.x = 0 (a triangle containing the x direction) contributes nothing; the
+    # values either side of 1e-6 are in test_oracle_thresholds.py
     grid = 8
     info = oracle.make_info([0, 0, 0], [8, 8, 8], grid, 1)
     tri = np.array([[[0.5, 1.2, 1.3], [7.5, 1.2, 1.3], [3.0, 6.2, 5.9]]], np.float32)
