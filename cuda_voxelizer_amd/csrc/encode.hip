@@ -6,8 +6,8 @@
 //
 //   1. permute   the linear table (a Morton table is converted first) into
 //                the writer's traversal order, as a bit array:
-//                  binvox      x outer, z, y fastest   (util_io.cpp:231-233)
-//                  OBJ         x outer, y, z fastest   (util_io.cpp:114-116, 164-166)
+//                  binvox      x outer, z, y fastest   (util_io.cpp:220-222)
+//                  OBJ         x outer, y, z fastest   (util_io.cpp:115-117, 169-171)
 //                32x32 bit blocks are transposed in registers across lanes.
 //   2. count     per 32-voxel word, the output bytes (text formats) or the
 //                run-length pairs (binvox) that start in the word;
@@ -15,7 +15,7 @@
 //   4. emit      every word writes its bytes at its offset.
 //
 // binvox runs: the reference writes (value, count) byte pairs, a new pair at
-// every value change and after 255 equal voxels (util_io.cpp:228-262).  So a
+// every value change and after 255 equal voxels (util_io.cpp:218-245).  So a
 // pair starts at every value change s and at s + 255k inside the run that
 // starts at s.  binvox works on tiles of 2048 words (8 per thread): the
 // last change position per tile, an exclusive max-scan of those (one
@@ -309,7 +309,7 @@ __device__ __forceinline__ uint8_t* put_uint(uint8_t* o, uint32_t v) {
     return o + d;
 }
 
-// the 12 faces of write_cube (util_io.cpp:76-92), relative vertex indices
+// the 12 faces of write_cube (util_io.cpp:45-90), relative vertex indices
 __constant__ char kCubeFaces[] =
     "f -1 -3 -4\nf -1 -4 -2\nf -4 -3 -6\nf -4 -6 -5\nf -3 -1 -8\nf -3 -8 -6\n"
     "f -1 -2 -7\nf -1 -7 -8\nf -2 -4 -5\nf -2 -5 -7\nf -5 -6 -8\nf -5 -8 -7\n";
@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(256) obj_kernel(const uint32_t* __restrict__ P
 }
 
 // ---------------------------------------------------------------------------
-// 2 + 4: MagicaVoxel .vox (write_vox, util_io.cpp:266-300, through the
+// 2 + 4: MagicaVoxel .vox (write_vox, util_io.cpp:249-285, through the
 // vendored VoxWriter, libs/magicavoxel_file_writer/VoxWriter.cpp)
 // ---------------------------------------------------------------------------
 // The writer is fed AddVoxel(x, G - z, y) in x, y, z order (z fastest) and
@@ -580,7 +580,7 @@ static vox_status grow(void*& p, uint64_t& have, uint64_t need) {
 // cube offsets use size_t differences, a float (cube - min + 0.5f) * 126,
 // then double subtractions of the voxel bounding box; note that the writer
 // updates minCubeY and minCubeZ as min(minCubeX, oy / oz of the latest
-// voxel) (VoxWriter.cpp:449-451), so they depend on the last voxel fed.
+// voxel) (VoxWriter.cpp:450-452), so they depend on the last voxel fed.
 // Float-to-int conversions outside the int range give INT_MIN, as the x86
 // truncating conversion does.  The writer leaves palette entry 255
 // uninitialised (VoxWriter.cpp:536-545); it is written as 0 here.

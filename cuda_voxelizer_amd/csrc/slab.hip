@@ -4,7 +4,7 @@
 // tile is 16 x 16 rows = 32 KB), one workgroup per tile.  A call is four
 // kernels on one stream and no host synchronisation:
 //   1. slab_count   per triangle: its clamped y/z extent -- surface: the voxel
-//                   box of cpu_voxelizer.cpp:71-80; solid: the column range of
+//                   box of cpu_voxelizer.cpp:74-80; solid: the column range of
 //                   :280-290 (valid triangles only) -- and so the tiles it
 //                   touches (at most kMaxTiles, else the triangle is "wide");
 //                   counts merged per workgroup in an LDS hash, then one global

@@ -114,7 +114,7 @@ __device__ __forceinline__ void edge_test(EdgeTest& E, const float v[3][3], cons
     }
 }
 
-// Clamped voxel box of a translated triangle (cpu_voxelizer.cpp:71-80).
+// Clamped voxel box of a translated triangle (cpu_voxelizer.cpp:74-80).
 __device__ __forceinline__ void surface_box(const float v[3][3], const vox_info& info, int lo[3], int hi[3]) {
 #pragma unroll
     for (int k = 0; k < 3; k++) {

@@ -16,9 +16,9 @@ write_vox              ``<file>_<G>.vox``                          MagicaVoxel c
 =====================  ==========================================  ==========================
 
 ``<file>`` is the input path including its extension (the reference passes
-``filename``, not ``filename_base``, main.cpp:238-249).  write_obj_cubes in
+``filename``, not ``filename_base``, main.cpp:244-258).  write_obj_cubes in
 the reference finally re-reads and re-writes its file with trimesh2
-(reorder_verts, util_io.cpp:134-146); that pass is not reproduced.  The
+(reorder_verts, util_io.cpp:139-149); that pass is not reproduced.  The
 MagicaVoxel writer leaves palette entry 255 uninitialised; it is 0 here.
 """
 from __future__ import annotations
@@ -107,7 +107,7 @@ def binvox_header(info) -> bytes:
 
 
 def write_binary(table, filename: str) -> str:
-    """write_binary (util_io.cpp:188-196): the raw table to ``<filename>.bin``."""
+    """write_binary (util_io.cpp:192-200): the raw table to ``<filename>.bin``."""
     import torch
     t = table.cpu() if isinstance(table, torch.Tensor) else torch.from_numpy(np.asarray(table))
     path = filename + ".bin"
@@ -124,24 +124,24 @@ def _write(path: str, head: bytes, body: bytes) -> str:
 
 
 def write_binvox(table, info, filename: str, morton: bool = False) -> str:
-    """write_binvox (util_io.cpp:214-263) -> ``<filename>_<G>.binvox``."""
+    """write_binvox (util_io.cpp:202-246) -> ``<filename>_<G>.binvox``."""
     G = info.grid
     return _write(f"{filename}_{G}.binvox", binvox_header(info), encode(table, G, "binvox", morton))
 
 
 def write_obj_pointcloud(table, info, filename: str, morton: bool = False) -> str:
-    """write_obj_pointcloud (util_io.cpp:149-186) -> ``<filename>_<G>_pointcloud.obj``."""
+    """write_obj_pointcloud (util_io.cpp:154-190) -> ``<filename>_<G>_pointcloud.obj``."""
     G = info.grid
     return _write(f"{filename}_{G}_pointcloud.obj", b"", encode(table, G, "obj_points", morton))
 
 
 def write_obj_cubes(table, info, filename: str, morton: bool = False) -> str:
-    """write_obj_cubes (util_io.cpp:95-147, without the trimesh2 re-write) -> ``<filename>_<G>_voxels.obj``."""
+    """write_obj_cubes (util_io.cpp:92-152, without the trimesh2 re-write) -> ``<filename>_<G>_voxels.obj``."""
     G = info.grid
     return _write(f"{filename}_{G}_voxels.obj", b"", encode(table, G, "obj", morton))
 
 
 def write_vox(table, info, filename: str, morton: bool = False) -> str:
-    """write_vox (util_io.cpp:266-300, VoxWriter) -> ``<filename>_<G>.vox``."""
+    """write_vox (util_io.cpp:249-285, VoxWriter) -> ``<filename>_<G>.vox``."""
     G = info.grid
     return _write(f"{filename}_{G}.vox", b"", encode(table, G, "vox", morton))

@@ -236,20 +236,20 @@ void vox_free_mesh(vox_mesh* mesh);
    scans and emits the file body on the device; the caller copies the bytes out.
      VOX_ENC_BINVOX      binvox RLE body: (value, count) byte pairs, y fastest,
                          then z, then x; a new pair after 255 equal voxels
-                         (write_binvox, util_io.cpp:214-263; header below)
+                         (write_binvox, util_io.cpp:202-246; header below)
      VOX_ENC_OBJ_POINTS  "v x+0.5 y+0.5 z+0.5" per set voxel, z fastest, then y,
-                         then x (write_obj_pointcloud, util_io.cpp:149-186)
+                         then x (write_obj_pointcloud, util_io.cpp:154-190)
      VOX_ENC_OBJ_CUBES   8 "v" + 12 "f" lines per set voxel in the same order
-                         (write_cube / write_obj_cubes, util_io.cpp:44-123; the
+                         (write_cube / write_obj_cubes, util_io.cpp:45-133; the
                          reference then re-writes the file through trimesh2's
                          reorder_verts, which is not reproduced)
    Tables are cubic, either layout (Morton needs a power-of-two grid >= 4),
    grid edge 1..2048.  The morton output of the reference (write_binary of the
-   raw table, util_io.cpp:188-196) needs no encoder: copy the table. */
+   raw table, util_io.cpp:192-200) needs no encoder: copy the table. */
 #define VOX_ENC_BINVOX 1
 #define VOX_ENC_OBJ_POINTS 2
 #define VOX_ENC_OBJ_CUBES 3
-#define VOX_ENC_VOX 4 /* complete MagicaVoxel .vox file (write_vox, util_io.cpp:266-300, through
+#define VOX_ENC_VOX 4 /* complete MagicaVoxel .vox file (write_vox, util_io.cpp:249-285, through
                          the vendored VoxWriter: 126^3 cubes, XYZI chunks, scene graph, palette) */
 typedef struct vox_encoder vox_encoder;
 /* An encoder bound to a stream; it keeps grow-only device buffers. */
@@ -264,7 +264,7 @@ vox_status vox_encoder_data(vox_encoder* enc, const uint8_t** d_bytes);
 /* Copies the first `bytes` bytes of the last encoded body to host memory. */
 vox_status vox_encoder_copy(vox_encoder* enc, void* host, uint64_t bytes);
 vox_status vox_encoder_destroy(vox_encoder* enc);
-/* The binvox ASCII header (util_io.cpp:219-226: dim, translate = bbox min,
+/* The binvox ASCII header (util_io.cpp:211-216: dim, translate = bbox min,
    scale = largest bbox extent, %g formatting) into buf; returns its length as
    snprintf does, or -1 if info is NULL. */
 int vox_binvox_header(const vox_info* info, char* buf, size_t cap);

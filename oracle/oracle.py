@@ -203,20 +203,20 @@ def _writer(name: str, table: np.ndarray, grid: int, morton: bool) -> bytes:
 
 
 def binvox_body(table: np.ndarray, grid: int, morton: bool = False) -> bytes:
-    """RLE body of write_binvox (util_io.cpp:228-262)."""
+    """RLE body of write_binvox (util_io.cpp:218-245)."""
     return _writer("voxo_binvox_body", table, grid, morton)
 
 
 def obj_points(table: np.ndarray, grid: int, morton: bool = False) -> bytes:
-    """Text of write_obj_pointcloud (util_io.cpp:149-186)."""
+    """Text of write_obj_pointcloud (util_io.cpp:154-190)."""
     return _writer("voxo_obj_points", table, grid, morton)
 
 
 def obj_cubes(table: np.ndarray, grid: int, morton: bool = False) -> bytes:
-    """Text of write_obj_cubes before its trimesh2 re-write (util_io.cpp:44-123)."""
+    """Text of write_obj_cubes before its trimesh2 re-write (util_io.cpp:45-133)."""
     return _writer("voxo_obj_cubes", table, grid, morton)
 
 
 def vox_file(table: np.ndarray, grid: int, morton: bool = False) -> bytes:
-    """MagicaVoxel file of write_vox (util_io.cpp:266-300, VoxWriter)."""
+    """MagicaVoxel file of write_vox (util_io.cpp:249-285, VoxWriter)."""
     return _writer("voxo_vox_file", table, grid, morton)

@@ -94,7 +94,7 @@ static inline f3 cross3(f3 a, f3 b) {
 }
 /* host rsqrtf = 1.0f / sqrtf(x) (helper_math.h:78-81); normalize = v * rsqrtf(dot(v,v)) */
 static inline f3 normalize3(f3 v) { float inv = 1.0f / sqrtf(dot3(v, v)); return muls3(v, inv); }
-/* host fminf / fmaxf are ternaries, not IEEE minNum (helper_math.h:56-64) */
+/* host fminf / fmaxf are ternaries, not IEEE minNum (helper_math.h:58-66) */
 static inline float hfminf(float a, float b) { return a < b ? a : b; }
 static inline float hfmaxf(float a, float b) { return a > b ? a : b; }
 static inline f3 fmin3(f3 a, f3 b) { return mk3(hfminf(a.x, b.x), hfminf(a.y, b.y), hfminf(a.z, b.z)); }

@@ -6,10 +6,10 @@
  * same per-voxel checkVoxel reads, the file bodies that the reference's
  * writers produce (Forceflow/cuda_voxelizer v0.6):
  *
- *   src/util_io.cpp:214-263  write_binvox          header + (value, count) RLE
- *   src/util_io.cpp:149-186  write_obj_pointcloud  "v x+0.5 y+0.5 z+0.5"
- *   src/util_io.cpp:44-93    write_cube            8 "v" + 12 "f" lines
- *   src/util_io.cpp:95-147   write_obj_cubes       loop order (before its
+ *   src/util_io.cpp:202-246  write_binvox          header + (value, count) RLE
+ *   src/util_io.cpp:154-190  write_obj_pointcloud  "v x+0.5 y+0.5 z+0.5"
+ *   src/util_io.cpp:45-90    write_cube            8 "v" + 12 "f" lines
+ *   src/util_io.cpp:92-152   write_obj_cubes       loop order (before its
  *                            trimesh2 reorder_verts re-write, not restated:
  *                            trimesh2 is not available here)
  *   src/util.h:25-38         checkVoxel            linear read-back
@@ -117,7 +117,7 @@ uint64_t voxo_obj_cubes(const uint32_t* table, uint32_t grid, int morton, uint8_
     return s.len;
 }
 
-/* ---- MagicaVoxel .vox (util_io.cpp:266-300 + libs/magicavoxel_file_writer) ----
+/* ---- MagicaVoxel .vox (util_io.cpp:249-285 + libs/magicavoxel_file_writer) ----
  * write_vox feeds AddVoxel(x, G - z, y, 1) for every set voxel in x, y, z
  * order (z fastest) to a VoxWriter with 126^3 cubes (VoxWriter.h default
  * arguments), after AddColor(255, 255, 255, 0, 0).  Restated here:
