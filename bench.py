@@ -215,7 +215,7 @@ def main() -> None:
     ap.add_argument("--config", default="torus1024", choices=sorted(CONFIGS) + list(MODES))
     ap.add_argument("--extra", default="sh1024s,torus512m",
                     help="comma-separated workloads reported as extra keys of the same line ('' for none)")
-    ap.add_argument("--algo", default="auto", choices=["auto", "tile", "chunk", "direct", "marker", "slab"])
+    ap.add_argument("--algo", default="auto", choices=["auto", "tile", "chunk", "direct", "marker", "slab", "region"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

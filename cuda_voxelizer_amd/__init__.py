@@ -36,7 +36,7 @@ __all__ = [
     "prepare_mesh",
 ]
 
-ALGOS = tuple(ALGO_FLAGS)  # "auto", "direct", "tile", "chunk", "marker"
+ALGOS = tuple(ALGO_FLAGS)  # "auto", "direct", "tile", "chunk", "marker", "slab", "region"
 
 
 def _algo_flag(algo: str) -> int:
@@ -225,13 +225,14 @@ def voxelize_surface(info: VoxInfo, tris, morton: bool = False, table=None, accu
 
     Without ``accumulate`` the table is fully defined by this call; with it
     bits are OR-ed into the existing contents (reference semantics).
-    ``tune``: VOX_TUNE_* bits (``_capi.tune_*``); every setting gives the same table."""
+    ``tune``: VOX_TUNE_* bits (``_capi.tune_*``) and the algorithm's VOX_SLAB_* / VOX_REGION_*
+    bits (``_capi.region_*``); every setting gives the same table."""
     torch = _torch()
     with _OnStream(stream, tris.device) as on:
         if table is None:
             table = torch.empty(table_words(info.grid), dtype=torch.int32, device=tris.device)
         _info_checked(info, tris, table)
-        flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | _algo_flag(algo) | (int(tune) & _capi.TUNE_MASK)
+        flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | _algo_flag(algo) | (int(tune) & _capi.KNOB_MASK)
         lib = _capi.load()
         check(lib.vox_surface(ctypes.byref(info._c), tris.data_ptr(), table.data_ptr(),
                               VOX_LAYOUT_MORTON if morton else VOX_LAYOUT_LINEAR, flags, on.ptr))
@@ -252,7 +253,7 @@ def voxelize_solid_table(info: VoxInfo, tris, morton: bool = False, table=None, 
         if table is None:
             table = torch.empty(table_words(info.grid), dtype=torch.int32, device=tris.device)
         _info_checked(info, tris, table)
-        flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | _algo_flag(algo) | (int(tune) & _capi.TUNE_MASK)
+        flags = (VOX_FLAG_ACCUMULATE if accumulate else 0) | _algo_flag(algo) | (int(tune) & _capi.KNOB_MASK)
         cl_ptr = None
         if offrow is not None:
             _check_tensor(offrow, "torch.int64", "offrow")
@@ -317,7 +318,7 @@ class Batch:
             it.d_offrow = j.offrow.data_ptr() if j.offrow is not None else None
             it.layout = VOX_LAYOUT_MORTON if j.morton else VOX_LAYOUT_LINEAR
             it.solid = 1 if j.solid else 0
-            it.flags = (VOX_FLAG_ACCUMULATE if j.accumulate else 0) | _algo_flag(j.algo) | (int(j.tune) & _capi.TUNE_MASK)
+            it.flags = (VOX_FLAG_ACCUMULATE if j.accumulate else 0) | _algo_flag(j.algo) | (int(j.tune) & _capi.KNOB_MASK)
         handle = ctypes.c_void_p()
         check(lib.vox_batch_create(arr, len(jobs), 0 if use_graph else _capi.VOX_BATCH_NO_GRAPH,
                                    ctypes.byref(handle)))

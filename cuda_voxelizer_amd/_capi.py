@@ -27,9 +27,25 @@ VOX_FLAG_ALGO_TILE = 0x4
 VOX_FLAG_ALGO_CHUNK = 0x8
 VOX_FLAG_ALGO_MARKER = 0x10
 VOX_FLAG_ALGO_SLAB = 0x20
+VOX_FLAG_ALGO_REGION = 0x40
 ALGO_FLAGS = {"auto": 0, "direct": VOX_FLAG_ALGO_DIRECT, "tile": VOX_FLAG_ALGO_TILE, "chunk": VOX_FLAG_ALGO_CHUNK,
-              "marker": VOX_FLAG_ALGO_MARKER, "slab": VOX_FLAG_ALGO_SLAB}
+              "marker": VOX_FLAG_ALGO_MARKER, "slab": VOX_FLAG_ALGO_SLAB, "region": VOX_FLAG_ALGO_REGION}
 
+
+
+def region_words(lg: int) -> int:
+    """VOX_REGION_WORDS: 2^lg table words per region (10..13)."""
+    return ((lg - 9) & 15) << 8
+
+
+def region_stage(lg: int) -> int:
+    """VOX_REGION_STAGE: 2^lg staged pairs per chunk of triangles (0..10)."""
+    return ((lg + 1) & 31) << 12
+
+
+def region_desc(lg: int) -> int:
+    """VOX_REGION_DESC: 2^lg descriptors per region (0..9)."""
+    return ((lg + 1) & 15) << 17
 
 
 def tune_big(lg: int) -> int:
@@ -48,6 +64,8 @@ def tune_tile(t: int) -> int:
 
 
 TUNE_MASK = 0xFF000000
+ALGO_KNOB_MASK = 0x007FFF00  # VOX_SLAB_* / VOX_REGION_* bits (read only with their algorithm flag)
+KNOB_MASK = TUNE_MASK | ALGO_KNOB_MASK
 
 
 _STATUS_NAMES = {0: "VOX_OK", 1: "VOX_ERR_INVALID", 2: "VOX_ERR_HIP", 3: "VOX_ERR_OOM",

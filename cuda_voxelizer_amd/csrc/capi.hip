@@ -162,7 +162,7 @@ vox_status run(const vox_info* info, const float* d_tris, uint32_t* d_table, int
     vox_status s = validate(info, d_tris, d_table, layout);
     if (s != VOX_OK) return s;
     const unsigned algo_bits = flags & (VOX_FLAG_ALGO_DIRECT | VOX_FLAG_ALGO_TILE | VOX_FLAG_ALGO_CHUNK |
-                                        VOX_FLAG_ALGO_MARKER | VOX_FLAG_ALGO_SLAB);
+                                        VOX_FLAG_ALGO_MARKER | VOX_FLAG_ALGO_SLAB | VOX_FLAG_ALGO_REGION);
     if (algo_bits & (algo_bits - 1)) return fail(VOX_ERR_INVALID, "at most one VOX_FLAG_ALGO_* flag");
     if (solid && (flags & VOX_FLAG_ALGO_CHUNK)) return fail(VOX_ERR_INVALID, "the chunk algorithm is surface-only");
     if (!solid && (flags & VOX_FLAG_ALGO_MARKER))
@@ -179,8 +179,10 @@ vox_status run(const vox_info* info, const float* d_tris, uint32_t* d_table, int
     const int wg = (int)((flags >> 21) & 3u);
     const bool have_slab = layout == VOX_LAYOUT_LINEAR && (((uintptr_t)d_table) & 15u) == 0 &&
                            make_slab_plan(G, SP, lg_ty, lg_tz, split_lg, wg);
-    enum { DIRECT, TILE, CHUNK, MARKER, SLAB } algo;
+    const bool have_region = region_supported(G, layout, solid);
+    enum { DIRECT, TILE, CHUNK, MARKER, SLAB, REGION } algo;
     if (flags & VOX_FLAG_ALGO_DIRECT) algo = DIRECT;
+    else if ((flags & VOX_FLAG_ALGO_REGION) && have_region) algo = REGION;
     else if (flags & VOX_FLAG_ALGO_SLAB) algo = have_slab ? SLAB : (have_plan ? TILE : DIRECT);
     else if (flags & VOX_FLAG_ALGO_TILE) algo = have_plan ? TILE : DIRECT;
     else if (flags & VOX_FLAG_ALGO_CHUNK) algo = chunk_supported(G) ? CHUNK : DIRECT;
@@ -190,6 +192,17 @@ vox_status run(const vox_info* info, const float* d_tris, uint32_t* d_table, int
     else if (have_plan && prefer_tile_surface(info->n_triangles, G)) algo = TILE;
     else algo = chunk_supported(G) ? CHUNK : (have_plan ? TILE : DIRECT);
     hipError_t e;
+    if (algo == REGION) {
+        const RegionKnobs K = region_knobs(flags);
+        if (solid)
+            e = launch_solid_region(*info, d_tris, d_table, layout, accumulate, (unsigned long long*)d_offrow, st,
+                                    tune_big(flags), K);
+        else
+            e = launch_surface_region(*info, d_tris, d_table, layout, accumulate, st, tune_big(flags),
+                                      tune_waves(flags), K);
+        if (e != hipSuccess) return hip_fail(e, "region kernels launch");
+        return VOX_OK;
+    }
     if (algo == SLAB) {
         void* ws = nullptr;
         SlabLayout L;

@@ -17,6 +17,7 @@
 // beforehand (unless accumulating).
 #include "jobs.h"
 #include "raster.h"
+#include "region.h"
 #include "vox_kernels.h"
 
 #pragma clang fp contract(off)
@@ -40,6 +41,7 @@ struct ChunkShared {
     union {
         float tri[kChunk * 9];  // staged vertices (until every lane has its setup)
         SurfRec recs[kRecs];    // then: records of the queued triangles of one round
+        RegionFlushShared flush;  // last: grouping of the region algorithm's pairs
     };
     unsigned long long incl[kRecs];  // inclusive prefix of the round's item counts
     unsigned wq[kChunk / 64];        // queued triangles per wave
@@ -51,11 +53,14 @@ static_assert(sizeof(SurfRec) * kRecs <= sizeof(float) * kChunk * 9, "records mu
 // found at its first probe) and for Morton keys (a chunk's words are a compact
 // range of brick codes); linear keys repeat modulo the table size with the z
 // stride, so they are hashed multiplicatively.
+// REG (region algorithm): more slots are tried, and a word that finds none is
+// staged as a fallback pair instead of going to the table.
+template <bool REG>
 __device__ __forceinline__ void word_or(ChunkShared& S, uint32_t* __restrict__ table, bool ident, uint32_t key,
-                                        uint32_t mask) {
+                                        uint32_t mask, const RegionArgs& RA) {
     uint32_t h = ident ? (key & (kHashCap - 1)) : (key * 2654435761u) >> (32 - kHashBits);
 #pragma unroll 1
-    for (int probe = 0; probe < kProbes; probe++) {
+    for (int probe = 0; probe < (REG ? kRegionProbes : kProbes); probe++) {
         uint32_t k = S.keys[h];
         if (k == kEmpty) {
             k = atomicCAS(&S.keys[h], kEmpty, key);
@@ -67,13 +72,18 @@ __device__ __forceinline__ void word_or(ChunkShared& S, uint32_t* __restrict__ t
         }
         h = (h + 1) & (kHashCap - 1);
     }
-    atomicOr(table + key, mask);  // hash table crowded: go to memory directly
+    if (REG)
+        region_fallback(RA.P, RA.W, key, mask);
+    else
+        atomicOr(table + key, mask);  // hash table crowded: go to memory directly
 }
 
 // Passing voxels of row (y, z) as a mask (bit k = voxel cx + k, cx a multiple
-// of 32) -> hash, in the table's layout.
+// of 32) -> hash, keyed by table word (REG, linear layout: by region row key).
+template <bool REG>
 __device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__ table, bool ident,
-                                          const vox_info& info, int layout, int cx, int y, int z, uint32_t mask) {
+                                          const vox_info& info, int layout, int cx, int y, int z, uint32_t mask,
+                                          const RegionArgs& RA) {
     if (!mask) return;
     const uint32_t G = info.gridsize[0];
     if (layout == VOX_LAYOUT_MORTON) {
@@ -82,10 +92,14 @@ __device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__
                                                       (spread10((uint32_t)z) << 2))
                                          : morton3((uint32_t)cx, (uint32_t)y, (uint32_t)z);
         const uint32_t yz_low = ((uint32_t)(y & 1) << 1) | ((uint32_t)(z & 1) << 2) | ((uint32_t)((y >> 1) & 1) << 4);
-        morton_chunk_words(base, yz_low, mask, [&](uint64_t w, uint32_t bits) { word_or(S, table, ident, (uint32_t)w, bits); });
+        morton_chunk_words(base, yz_low, mask,
+                           [&](uint64_t w, uint32_t bits) { word_or<REG>(S, table, ident, (uint32_t)w, bits, RA); });
+    } else if (REG) {
+        word_or<REG>(S, table, ident, region_row_key(RA.P, (uint32_t)cx >> 5, (uint32_t)y, (uint32_t)z),
+                     __brev(mask), RA);
     } else if ((G & 31u) == 0) {
         const uint64_t w = ((uint64_t)cx + (uint64_t)y * G + (uint64_t)z * G * G) >> 5;
-        word_or(S, table, ident, (uint32_t)w, __brev(mask));
+        word_or<REG>(S, table, ident, (uint32_t)w, __brev(mask), RA);
     } else {
         // linear rows not aligned to words: per voxel
         const uint64_t row = (uint64_t)y * G + (uint64_t)z * G * G;
@@ -96,13 +110,13 @@ __device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__
             const uint64_t idx = row + (uint64_t)(cx + k);
             const uint32_t w = (uint32_t)(idx >> 5);
             if (w != cur) {
-                if (bits) word_or(S, table, ident, cur, bits);
+                if (bits) word_or<REG>(S, table, ident, cur, bits, RA);
                 cur = w;
                 bits = 0;
             }
             bits |= bit_mask(idx);
         }
-        if (bits) word_or(S, table, ident, cur, bits);
+        if (bits) word_or<REG>(S, table, ident, cur, bits, RA);
     }
 }
 
@@ -111,8 +125,10 @@ __device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__
 // a row then tests the XY edges and the plane only at the x that passed ZX.
 // Every test is the same binary32 expression as in vox_device.h, evaluated
 // for a subset of the voxels: the resulting bits are unchanged.
+template <bool REG>
 __device__ __forceinline__ void lane_raster(ChunkShared& S, uint32_t* __restrict__ table, bool ident,
-                                            const SurfTri& T, const vox_info& info, int layout) {
+                                            const SurfTri& T, const vox_info& info, int layout,
+                                            const RegionArgs& RA) {
     const float ux = info.unit[0], uy = info.unit[1], uz = info.unit[2];
     const int xa = T.lo[0], xb = T.hi[0];
     const int c0 = xa & ~31, sh = xa - c0;
@@ -120,7 +136,7 @@ __device__ __forceinline__ void lane_raster(ChunkShared& S, uint32_t* __restrict
     // computed once per lane, row and slice instead of once per emitted mask
     const bool m32 = layout == VOX_LAYOUT_MORTON && info.gridsize[0] <= 1024u;
     const uint32_t sc0 = m32 ? spread10((uint32_t)c0) : 0u, sc1 = m32 ? spread10((uint32_t)c0 + 32u) : 0u;
-    const auto put = [&](uint64_t w, uint32_t bits) { word_or(S, table, ident, (uint32_t)w, bits); };
+    const auto put = [&](uint64_t w, uint32_t bits) { word_or<REG>(S, table, ident, (uint32_t)w, bits, RA); };
     for (int z = T.lo[2]; z <= T.hi[2]; z++) {
         const float pz = (float)z * uz;
         float azx[3];
@@ -177,8 +193,8 @@ __device__ __forceinline__ void lane_raster(ChunkShared& S, uint32_t* __restrict
                 if (M >> 32) morton_chunk_words(sc1 | syz, yz_low, (uint32_t)(M >> 32), put);
                 continue;
             }
-            emit_mask(S, table, ident, info, layout, c0, y, z, (uint32_t)M);
-            if (M >> 32) emit_mask(S, table, ident, info, layout, c0 + 32, y, z, (uint32_t)(M >> 32));
+            emit_mask<REG>(S, table, ident, info, layout, c0, y, z, (uint32_t)M, RA);
+            if (M >> 32) emit_mask<REG>(S, table, ident, info, layout, c0 + 32, y, z, (uint32_t)(M >> 32), RA);
         }
     }
 }
@@ -199,13 +215,15 @@ __device__ __forceinline__ void staged_vertices(const ChunkShared& S, int lt, co
 // slots follow lane order, so every part sees the same items).  MULTI: the job
 // comes from the launch's job map; otherwise it is `one` (kernel arguments).
 // LOOP: some job has span > 1 (otherwise the chunk loop is compiled out, which
-// keeps the register count of the one-chunk form down).
+// keeps the register count of the one-chunk form down).  REG: the merged
+// words are staged as region pairs (RA) instead of ORed into the table.
 // Workgroups per CU: the LDS allows six.  The bound holds the registers at
 // five waves per SIMD, or six with WIDE (a few spilled registers: it pays for
 // the linear layout, whose chunks touch more words and so wait longer on the
 // LDS table, not for the Morton layout), or four for the chunk loop.
-template <bool MULTI, bool LOOP, bool WIDE>
-__global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5)) surface_chunk_kernel(ItemDev one, ItemMap M) {
+template <bool MULTI, bool LOOP, bool WIDE, bool REG>
+__global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5))
+    surface_chunk_kernel(ItemDev one, ItemMap M, RegionArgs RA) {
     __shared__ ChunkShared S;
     __shared__ BlockJob J;
     uint32_t local = blockIdx.x;
@@ -227,7 +245,7 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5)) surface_chu
     const uint64_t end = n - begin > (uint64_t)span * kChunk ? begin + (uint64_t)span * kChunk : n;
     const uint32_t G = info.gridsize[0];
     const uint64_t words = ((uint64_t)G * G * G + 31) / 32;
-    const bool dense = words <= (uint64_t)kHashCap;
+    const bool dense = !REG && words <= (uint64_t)kHashCap;  // region keys are not table words
     const bool ident = dense || layout == VOX_LAYOUT_MORTON;
     for (int i = threadIdx.x; i < kHashCap; i += kChunk) {
         S.keys[i] = dense && (uint64_t)i < words ? (uint32_t)i : kEmpty;
@@ -256,7 +274,7 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5)) surface_chu
             float v[3][3];
             staged_vertices(S, threadIdx.x, info, v);
             surface_setup_tests(v, info, T);  // the box is T's already
-            lane_raster(S, table, ident, T, info, layout);  // nx <= 32 here
+            lane_raster<REG>(S, table, ident, T, info, layout, RA);  // nx <= 32 here
         }
         __syncthreads();  // S.tri is dead from here on; S.wq is complete
         int myq = -1, nq = 0;  // this lane's queue slot (lane order), queue length
@@ -330,7 +348,8 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5)) surface_chu
                 const int cx = ((bx0 >> 5) + k) << 5;
                 if (cx > bx1) continue;
                 const int x0 = cx > bx0 ? cx : bx0, x1 = (cx + 31) < bx1 ? (cx + 31) : bx1;
-                emit_mask(S, table, ident, info, layout, cx, y, z, row_mask(R.n, R.d1, R.d2, R.xy, R.zx, ux, py, pz, x0, x1, cx));
+                emit_mask<REG>(S, table, ident, info, layout, cx, y, z,
+                               row_mask(R.n, R.d1, R.d2, R.xy, R.zx, ux, py, pz, x0, x1, cx), RA);
             }
             __syncthreads();  // the records are rewritten by the next round
         }
@@ -346,9 +365,13 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5)) surface_chu
         kk[u] = S.keys[threadIdx.x + u * kChunk];
         vv[u] = S.vals[threadIdx.x + u * kChunk];
     }
+    if (REG) {
+        region_flush<kChunk, kPer>(S.flush, kk, vv, S.keys, S.vals, RA.P, RA.W);
+    } else {
 #pragma unroll
-    for (int u = 0; u < kPer; u++)
-        if (kk[u] != kEmpty && vv[u] != 0u) atomicOr(table + kk[u], vv[u]);
+        for (int u = 0; u < kPer; u++)
+            if (kk[u] != kEmpty && vv[u] != 0u) atomicOr(table + kk[u], vv[u]);
+    }
 }
 
 // Zero-fill with 16-byte stores (a kernel, so it can live inside a captured
@@ -424,18 +447,19 @@ hipError_t launch_chunk_jobs(const ItemDev& one, const ItemMap& M, uint32_t unit
                              int waves) {
     if (units == 0) return hipSuccess;
     const bool wide = waves == 6 || (waves == 0 && !M.items && one.layout == VOX_LAYOUT_LINEAR);
+    const RegionArgs none{};
     if (M.items && loop)
-        surface_chunk_kernel<true, true, false><<<units, kChunk, 0, stream>>>(one, M);
+        surface_chunk_kernel<true, true, false, false><<<units, kChunk, 0, stream>>>(one, M, none);
     else if (M.items && wide)
-        surface_chunk_kernel<true, false, true><<<units, kChunk, 0, stream>>>(one, M);
+        surface_chunk_kernel<true, false, true, false><<<units, kChunk, 0, stream>>>(one, M, none);
     else if (M.items)
-        surface_chunk_kernel<true, false, false><<<units, kChunk, 0, stream>>>(one, M);
+        surface_chunk_kernel<true, false, false, false><<<units, kChunk, 0, stream>>>(one, M, none);
     else if (loop)
-        surface_chunk_kernel<false, true, false><<<units, kChunk, 0, stream>>>(one, M);
+        surface_chunk_kernel<false, true, false, false><<<units, kChunk, 0, stream>>>(one, M, none);
     else if (wide)
-        surface_chunk_kernel<false, false, true><<<units, kChunk, 0, stream>>>(one, M);
+        surface_chunk_kernel<false, false, true, false><<<units, kChunk, 0, stream>>>(one, M, none);
     else
-        surface_chunk_kernel<false, false, false><<<units, kChunk, 0, stream>>>(one, M);
+        surface_chunk_kernel<false, false, false, false><<<units, kChunk, 0, stream>>>(one, M, none);
     return hipGetLastError();
 }
 
@@ -453,6 +477,25 @@ static ItemDev single_job(const vox_info& info, const float* tris, uint32_t* tab
 hipError_t launch_fill_zero(const vox_info& info, uint32_t* table, hipStream_t stream) {
     const ItemMap none{nullptr, nullptr, 1, 0};
     return launch_fill_jobs(single_job(info, nullptr, table, 0), none, fill_units(info.gridsize[0]), stream);
+}
+
+// Phase 1 of the region algorithm: one chunk of kChunk triangles per
+// workgroup, its merged words staged as region pairs (no table writes).
+hipError_t launch_surface_region_chunks(const vox_info& info, const float* tris, int layout, const RegionArgs& RA,
+                                        hipStream_t stream, uint32_t big, int waves) {
+    if (info.n_triangles == 0) return hipSuccess;
+    const ItemMap none{nullptr, nullptr, 1, 0};
+    ItemDev one = single_job(info, tris, nullptr, layout);
+    one.span = 1;
+    one.parts = 1;
+    one.big = big;
+    const uint32_t units = chunk_units(info.n_triangles, 1, 1);
+    const bool wide = waves == 6;  // six waves per SIMD (knob); built-in: five
+    if (wide)
+        surface_chunk_kernel<false, false, true, true><<<units, kChunk, 0, stream>>>(one, none, RA);
+    else
+        surface_chunk_kernel<false, false, false, true><<<units, kChunk, 0, stream>>>(one, none, RA);
+    return hipGetLastError();
 }
 
 hipError_t launch_surface_chunk(const vox_info& info, const float* tris, uint32_t* table, int layout,

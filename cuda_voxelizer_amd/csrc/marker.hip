@@ -25,7 +25,9 @@
 //
 // G % 32 == 0 (rows are whole words); Morton: power-of-two G up to 4096.
 // Per-column arithmetic: vox_device.h (the reference CPU path bit for bit).
+#include "bricks.h"
 #include "jobs.h"
+#include "region.h"
 #include "vox_device.h"
 #include "vox_host.h"
 #include "vox_kernels.h"
@@ -56,16 +58,54 @@ struct MarkerWs {
     unsigned long long* spill;  // [2*i] job << kJobShift | first bit, [2*i+1] last bit (inclusive)
 };
 
+// Region algorithm: the workgroup's markers, merged by XOR in an LDS hash
+// keyed in the region key space, then flushed as region pairs.
+constexpr int kMHashBits = 11;
+constexpr int kMHash = 1 << kMHashBits;
+template <bool REG>
+struct MarkerLds {};
+template <>
+struct MarkerLds<true> {
+    uint32_t keys[kMHash];
+    uint32_t vals[kMHash];
+    RegionFlushShared flush;
+};
+
+__device__ __forceinline__ void marker_xor(MarkerLds<true>& H, uint32_t key, uint32_t mask, const RegionArgs& RA) {
+    uint32_t h = (key * 2654435761u) >> (32 - kMHashBits);
+#pragma unroll 1
+    for (int probe = 0; probe < kRegionProbes; probe++) {
+        uint32_t k = H.keys[h];
+        if (k == kRegionEmpty) {
+            k = atomicCAS(&H.keys[h], kRegionEmpty, key);
+            if (k == kRegionEmpty) k = key;
+        }
+        if (k == key) {
+            atomicXor(&H.vals[h], mask);
+            return;
+        }
+        h = (h + 1) & (kMHash - 1);
+    }
+    region_fallback(RA.P, RA.W, key, mask);
+}
+
 // One hit column: marker toggle (and spill record) of the run x = 0..xmax.
+// REG: the marker goes to the LDS hash H instead of the table.
+template <bool REG>
 __device__ __forceinline__ void column_run(uint32_t* __restrict__ table, int64_t xmax, int y, int z, uint32_t G,
                                            uint64_t total_bits, const MarkerWs& ws, unsigned long long& offrow,
-                                           bool spills, unsigned long long jobtag) {
+                                           bool spills, unsigned long long jobtag, MarkerLds<REG>& H,
+                                           const RegionArgs& RA) {
     if (xmax < 0 || xmax >= (int64_t)G) offrow++;
     if (xmax < 0) return;
     const uint64_t row = (uint64_t)y * G + (uint64_t)z * G * G;
     const uint32_t xm = xmax < (int64_t)G ? (uint32_t)xmax : G - 1;
-    const uint64_t idx = row + xm;
-    atomicXor(table + (idx >> 5), bit_mask(idx));
+    if constexpr (REG) {
+        marker_xor(H, region_row_key(RA.P, xm >> 5, (uint32_t)y, (uint32_t)z), 1u << (31u - (xm & 31u)), RA);
+    } else {
+        const uint64_t idx = row + xm;
+        atomicXor(table + (idx >> 5), bit_mask(idx));
+    }
     if (spills && xmax >= (int64_t)G && row + G < total_bits) {
         // bits row+G .. row+xmax, clipped to the table
         const uint64_t last = (uint64_t)xmax < total_bits - 1 - row ? row + (uint64_t)xmax : total_bits - 1;
@@ -81,13 +121,16 @@ __device__ __forceinline__ void column_run(uint32_t* __restrict__ table, int64_t
 
 // Work unit: kMBlock consecutive triangles of one job; the markers go to the
 // job's table.  MULTI: the job comes from the launch's job map (block
-// prefix); otherwise it is `one`.
-template <bool MULTI>
-__global__ void __launch_bounds__(kMBlock) solid_marker_kernel(ItemDev one, ItemMap M, MarkerWs ws, int spills) {
+// prefix); otherwise it is `one`.  REG: the markers are staged as region
+// pairs (RA) instead (phase 1 of the region algorithm).
+template <bool MULTI, bool REG>
+__global__ void __launch_bounds__(kMBlock)
+    solid_marker_kernel(ItemDev one, ItemMap M, MarkerWs ws, int spills, RegionArgs RA) {
     __shared__ float s_tri[kMBlock * 9];
     __shared__ unsigned queue[kMQueue];
     __shared__ unsigned qn;
     __shared__ BlockJob J;
+    __shared__ MarkerLds<REG> H;
     uint32_t local = blockIdx.x;
     int job = 0;
     if (MULTI) {
@@ -109,6 +152,12 @@ __global__ void __launch_bounds__(kMBlock) solid_marker_kernel(ItemDev one, Item
     const int G = (int)info.gridsize[0];
     const uint64_t total_bits = (uint64_t)G * G * G;
     if (threadIdx.x == 0) qn = 0;
+    if constexpr (REG) {
+        for (int i = threadIdx.x; i < kMHash; i += kMBlock) {
+            H.keys[i] = kRegionEmpty;
+            H.vals[i] = 0u;
+        }
+    }
     const float* src = tris + first * 9;
     stage_floats<kMBlock>(s_tri, src, count * 9);
     __syncthreads();
@@ -148,8 +197,8 @@ __global__ void __launch_bounds__(kMBlock) solid_marker_kernel(ItemDev one, Item
                     for (int z = zlo; z <= zhi; z++) {
                         const float pz = ((float)z + 0.5f) * info.unit[2];
                         if (!column_hit(S, py, pz)) continue;
-                        column_run(table, column_xmax(S, info, py, pz), y, z, (uint32_t)G, total_bits, ws, offrow,
-                                   spills != 0, jobtag);
+                        column_run<REG>(table, column_xmax(S, info, py, pz), y, z, (uint32_t)G, total_bits, ws,
+                                        offrow, spills != 0, jobtag, H, RA);
                     }
                 }
             }
@@ -178,21 +227,22 @@ __global__ void __launch_bounds__(kMBlock) solid_marker_kernel(ItemDev one, Item
             const float py = ((float)y + 0.5f) * info.unit[1];
             const float pz = ((float)z + 0.5f) * info.unit[2];
             if (!column_hit(S, py, pz)) continue;
-            column_run(table, column_xmax(S, info, py, pz), y, z, (uint32_t)G, total_bits, ws, offrow,
-                       spills != 0, jobtag);
+            column_run<REG>(table, column_xmax(S, info, py, pz), y, z, (uint32_t)G, total_bits, ws, offrow,
+                            spills != 0, jobtag, H, RA);
         }
     }
     if (offrow_out && offrow) atomicAdd(offrow_out, offrow);
-}
-
-// In-word suffix along x: bit b (x = 31 - b) becomes the XOR of bits 0..b.
-__device__ __forceinline__ uint32_t prefix_xor_up(uint32_t w) {
-    w ^= w << 1;
-    w ^= w << 2;
-    w ^= w << 4;
-    w ^= w << 8;
-    w ^= w << 16;
-    return w;
+    if constexpr (REG) {
+        __syncthreads();  // every marker of the workgroup is in the hash
+        constexpr int kPer = kMHash / kMBlock;
+        uint32_t kk[kPer], vv[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            kk[u] = H.keys[threadIdx.x + u * kMBlock];
+            vv[u] = H.vals[threadIdx.x + u * kMBlock];
+        }
+        region_flush<kMBlock, kPer>(H.flush, kk, vv, H.keys, H.vals, RA.P, RA.W);
+    }
 }
 
 // Rows r_begin..r_end-1 of rw = G/32 words, in place; waves `wave` of
@@ -405,156 +455,19 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* marke
         }
     }
     __syncthreads();
-    const int lane = __lane_id(), wave = threadIdx.x >> 6;
-    if (!SUFFIX || fused) {
-        // rows stay as loaded (and suffixed)
-    } else if (rw <= 64) {
-        const uint32_t per = 64 / rw;
-        const uint32_t r_in = (uint32_t)lane / rw, j = (uint32_t)lane % rw;
-        const bool active_lane = r_in < per;
-        // four row groups per step: their LDS reads are issued before any result is written
-        for (uint32_t r00 = (uint32_t)wave * per; r00 < nrows; r00 += 16 * per) {
-            uint32_t w[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint32_t r = r00 + 4 * per * (uint32_t)u + r_in;
-                w[u] = active_lane && r < nrows ? tile[r * ts + j] : 0u;
+    if (SUFFIX && !fused) {
+        tile_suffix(tile, ts, rw, nrows, [&](uint32_t r, uint32_t j, uint32_t fin) {
+            if (linear) {
+                const uint64_t idx = ((uint64_t)(z0 + r / 32u) * G + y0 + (r & 31u)) * rw + j;
+                table[idx] = accumulate ? (table[idx] ^ fin) : fin;
+            } else {
+                tile[r * ts + j] = fin;
             }
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint32_t r = r00 + 4 * per * (uint32_t)u + r_in;
-                const bool act = active_lane && r < nrows;
-                const unsigned long long par = __ballot(act && (__popc(w[u]) & 1));
-                const uint32_t seg0 = r_in * rw;
-                const unsigned long long above =
-                    (active_lane && j + 1 < rw) ? ((par >> (seg0 + j + 1)) & ((1ull << (rw - 1 - j)) - 1ull))
-                                                : 0ull;
-                const uint32_t fin = prefix_xor_up(w[u]) ^ ((__popcll(above) & 1) ? 0xFFFFFFFFu : 0u);
-                if (act) {
-                    if (linear) {
-                        const uint64_t idx = ((uint64_t)(z0 + r / 32u) * G + y0 + (r & 31u)) * rw + j;
-                        table[idx] = accumulate ? (table[idx] ^ fin) : fin;
-                    } else {
-                        tile[r * ts + j] = fin;
-                    }
-                }
-            }
-        }
-    } else {
-        for (uint32_t r = (uint32_t)wave; r < nrows; r += 4) {
-            uint32_t carry = 0;
-            for (int64_t hi = (int64_t)rw; hi > 0; hi -= 64) {
-                const int64_t lo = hi > 64 ? hi - 64 : 0;
-                const bool act = (int64_t)lane < hi - lo;
-                const uint32_t j = (uint32_t)lo + (uint32_t)lane;
-                const uint32_t w = act ? tile[r * ts + j] : 0u;
-                const unsigned long long par = __ballot(act && (__popc(w) & 1));
-                const unsigned long long above = lane < 63 ? (par >> (lane + 1)) : 0ull;
-                const uint32_t fin = prefix_xor_up(w) ^ (((__popcll(above) + carry) & 1) ? 0xFFFFFFFFu : 0u);
-                carry = (uint32_t)((__popcll(par) + carry) & 1);
-                if (act) {
-                    if (linear) {
-                        const uint64_t idx = ((uint64_t)(z0 + r / 32u) * G + y0 + (r & 31u)) * rw + j;
-                        table[idx] = accumulate ? (table[idx] ^ fin) : fin;
-                    } else {
-                        tile[r * ts + j] = fin;
-                    }
-                }
-            }
-        }
+        });
     }
     if (linear) return;
     __syncthreads();
-    if (rz == 8u && (((uintptr_t)table) & 15u) == 0) {
-        // 32 y x 8 z rows: the block's Morton words, with the code bits above
-        // bit 5 fixed by (y0, z0), vary in code bits 5..10 (z1 x2 y2 z2 x3 y3),
-        // 12, 13 (x4 y4) and 3k for x_k, k >= 5.  A thread takes 8 words that
-        // are consecutive in memory (code bits 5..7 = z1 x2 y2) and stores them
-        // as two 16-byte writes; its index supplies z2 x3 y3 x4 y4 x5.. , so
-        // eight neighbouring lanes write 256 contiguous bytes.  Its 8 words
-        // draw on 32 rows (y0 y1 y2 z0 z1 free) and one byte (8 x) of each.
-        const uint64_t base_w = morton3(0u, y0, z0) >> 5;
-        for (uint32_t t = threadIdx.x; t < G; t += 256) {
-            const uint32_t z2 = t & 1u, x3 = (t >> 1) & 1u, y3 = (t >> 2) & 1u, x4 = (t >> 3) & 1u,
-                           y4 = (t >> 4) & 1u, xh = t >> 5;
-            const uint32_t xbyte = x3 | (x4 << 1) | (xh << 2);  // x >> 3
-            const uint32_t j = xbyte >> 2, bsh = 24u - 8u * (xbyte & 3u);
-            uint32_t bytes[32];  // row (y0 y1 y2 z0 z1) -> its byte of x
-#pragma unroll
-            for (int q = 0; q < 32; q++) {
-                const uint32_t y_l = ((uint32_t)q & 7u) | (y3 << 3) | (y4 << 4);
-                const uint32_t z_l = (((uint32_t)q >> 3) & 3u) | (z2 << 2);
-                bytes[q] = (tile[(z_l * 32u + y_l) * ts + j] >> bsh) & 0xFFu;
-            }
-            // The 256 bits (row q = y0 y1 y2 z0 z1, x offset xo = x0 x1 x2 within the byte, MSB
-            // first) become the 8 words k = z1 | x2 << 1 | y2 << 2, bit 31 - c with
-            // c = x0 | y0 << 1 | z0 << 2 | x1 << 3 | y1 << 4.  Word w = y2 | z0 << 1 | z1 << 2 packs
-            // the bytes of rows (y0, y1) = (0,0) (1,0) (0,1) (1,1) from the top, so bit p of it
-            // has p0 = !x0, p1 = !x1, p2 = !x2, p3 = !y0, p4 = !y1.  Exchanging the low nibbles
-            // of word pair (z0 = 0, 1) with the high ones puts x2 in the word index (z0 = 0
-            // keeps x2 = 0) and !z0 at p2; swapping index bits p1 and p3 (a delta swap) leaves
-            // p = 31 - c.
-            uint32_t W[8];
-#pragma unroll
-            for (int w = 0; w < 8; w++)
-                W[w] = (bytes[4 * w] << 24) | (bytes[4 * w + 1] << 16) | (bytes[4 * w + 2] << 8) | bytes[4 * w + 3];
-#pragma unroll
-            for (int w = 0; w < 8; w++) {
-                if (w & 2) continue;
-                const uint32_t t = (W[w] ^ (W[w | 2] >> 4)) & 0x0F0F0F0Fu;
-                W[w] ^= t;
-                W[w | 2] ^= t << 4;
-            }
-            uint32_t out[8];
-#pragma unroll
-            for (int w = 0; w < 8; w++) {
-                const uint32_t t = ((W[w] >> 6) ^ W[w]) & 0x00CC00CCu;
-                out[((w >> 2) & 1) | (((w >> 1) & 1) << 1) | ((w & 1) << 2)] = W[w] ^ t ^ (t << 6);
-            }
-            // word offset inside the block's code range: bits 3 4 5 7 8 from t, then x5.. at 10, 13, ...
-            uint64_t wl = ((uint64_t)z2 << 3) | ((uint64_t)x3 << 4) | ((uint64_t)y3 << 5) | ((uint64_t)x4 << 7) |
-                          ((uint64_t)y4 << 8);
-            for (uint32_t b = 0; (xh >> b) != 0u; b++) wl |= (uint64_t)((xh >> b) & 1u) << (10u + 3u * b);
-            uint4* dst = (uint4*)(table + base_w + wl);
-            uint4 lo = make_uint4(out[0], out[1], out[2], out[3]), hi = make_uint4(out[4], out[5], out[6], out[7]);
-            if (accumulate) {
-                const uint4 a = dst[0], b = dst[1];
-                lo.x ^= a.x; lo.y ^= a.y; lo.z ^= a.z; lo.w ^= a.w;
-                hi.x ^= b.x; hi.y ^= b.y; hi.z ^= b.z; hi.w ^= b.w;
-            }
-            dst[0] = lo;
-            dst[1] = hi;
-        }
-        return;
-    }
-    const uint32_t groups = 8u * (rz / 2u);  // brick rows (4 y x 2 z) in the block
-    for (uint32_t t = threadIdx.x; t < groups * rw; t += 256) {
-        const uint32_t j = t % rw, br = t / rw;
-        const uint32_t yb = br % 8u, zb = br / 8u;
-        uint32_t row_w[8];  // row q = (y1 << 2 | z0 << 1 | y0) -> its linear word j
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const uint32_t y0b = (uint32_t)q & 1u, z0b = ((uint32_t)q >> 1) & 1u, y1b = ((uint32_t)q >> 2) & 1u;
-            row_w[q] = tile[((zb * 2u + z0b) * 32u + yb * 4u + y0b + 2u * y1b) * ts + j];
-        }
-#pragma unroll
-        for (int xg = 0; xg < 8; xg++) {
-            uint32_t word = 0;
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const uint32_t y0b = (uint32_t)q & 1u, z0b = ((uint32_t)q >> 1) & 1u, y1b = ((uint32_t)q >> 2) & 1u;
-                // nibble bits 3..0 = x offsets 0..3; offset o = x0 + 2 x1 goes to code bit c0 | x0 | x1 << 3,
-                // i.e. word bit 31 - c0 - x0 - 8 x1: offsets 0, 1 are the adjacent word bits 31-c0, 30-c0
-                // and offsets 2, 3 the adjacent bits 23-c0, 22-c0 (c0 is a compile-time constant here)
-                const uint32_t nib = (row_w[q] >> (28 - 4 * xg)) & 0xFu;
-                const uint32_t c0 = (y0b << 1) | (z0b << 2) | (y1b << 4);
-                word |= ((nib >> 2) << (30u - c0)) | ((nib & 3u) << (22u - c0));
-            }
-            const uint64_t code = morton3(j * 32u + (uint32_t)xg * 4u, y0 + yb * 4u, z0 + zb * 2u);
-            uint32_t* dst = table + (code >> 5);
-            *dst = accumulate ? (*dst ^ word) : word;
-        }
-    }
+    tile_to_bricks(tile, ts, table, G, rw, rz, y0, z0, accumulate);
 }
 
 // Spilled run parts, XORed into the final tables: from the records (each
@@ -799,7 +712,8 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
     fin.table = table;
     if (info.n_triangles) {
         const uint64_t blocks = (info.n_triangles + kMBlock - 1) / kMBlock;
-        solid_marker_kernel<false><<<(unsigned)blocks, kMBlock, 0, stream>>>(one, none, ws, linear ? 1 : 0);
+        solid_marker_kernel<false, false><<<(unsigned)blocks, kMBlock, 0, stream>>>(one, none, ws, linear ? 1 : 0,
+                                                                                    RegionArgs{});
     }
     const uint32_t rz = linear ? 1u : (G <= 1024u ? 8u : 2u);
     const uint32_t nblocks = (G / 32u) * (G / rz);
@@ -812,6 +726,58 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
         g_mws[{dev, stream}].scratch_dirty = false;  // every row of the G^3 table was read and cleared
     }
     if (linear) spill_kernel<<<kSpillBlocks, 256, 0, stream>>>(fin, none, none, ws);
+    return hipGetLastError();
+}
+
+static hipError_t marker_ws_for(hipStream_t stream, MarkerWs& ws) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(g_mws_mutex);
+    MarkerBuffers& b = g_mws[{dev, stream}];
+    if (!b.ws) {
+        e = hipMalloc(&b.ws, 256 + 16ull * kSpillCap);
+        if (e == hipSuccess) e = hipMemset(b.ws, 0, 256);  // the counters start at zero
+        if (e != hipSuccess) return e;
+    }
+    ws = MarkerWs{(unsigned*)b.ws, (unsigned long long*)((char*)b.ws + 256)};
+    return hipSuccess;
+}
+
+// Region algorithm, solid: markers staged per region (phase 1), the build
+// pass (XOR merge, row suffix, one write per word, Morton bricks for the
+// Morton layout), then the spilled runs of the linear layout.
+hipError_t launch_solid_region(const vox_info& info, const float* tris, uint32_t* table, int layout,
+                               bool accumulate, unsigned long long* offrow, hipStream_t stream, uint32_t big,
+                               const RegionKnobs& K) {
+    RegionPlan P;
+    if (!make_region_plan(info.gridsize[0], layout, true, P, K)) return hipErrorInvalidValue;
+    RegionWs W;
+    hipError_t e = region_workspace(P, (info.n_triangles + kMBlock - 1) / kMBlock, stream, W);
+    if (e != hipSuccess) return e;
+    MarkerWs ws;
+    e = marker_ws_for(stream, ws);
+    if (e != hipSuccess) return e;
+    const bool linear = layout == VOX_LAYOUT_LINEAR;
+    const ItemMap none{nullptr, nullptr, 1, 0};
+    ItemDev one{};
+    one.big = big;
+    one.info = info;
+    one.tris = tris;
+    one.table = table;  // spilled runs go to the final table
+    one.offrow = offrow;
+    one.layout = layout;
+    one.solid = 1;
+    if (info.n_triangles) {
+        const uint64_t blocks = (info.n_triangles + kMBlock - 1) / kMBlock;
+        solid_marker_kernel<false, true><<<(unsigned)blocks, kMBlock, 0, stream>>>(one, none, ws, linear ? 1 : 0,
+                                                                                  RegionArgs{P, W});
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    e = launch_region_build(P, W, table, accumulate, stream);
+    if (e != hipSuccess) return e;
+    if (linear) spill_kernel<<<kSpillBlocks, 256, 0, stream>>>(one, none, none, ws);
     return hipGetLastError();
 }
 
@@ -834,7 +800,7 @@ hipError_t launch_marker_in_place(const vox_info& info, const float* tris, uint3
     one.solid = 1;
     if (info.n_triangles) {
         const uint64_t blocks = (info.n_triangles + kMBlock - 1) / kMBlock;
-        solid_marker_kernel<false><<<(unsigned)blocks, kMBlock, 0, stream>>>(one, none, ws, 1);
+        solid_marker_kernel<false, false><<<(unsigned)blocks, kMBlock, 0, stream>>>(one, none, ws, 1, RegionArgs{});
     }
     if ((G & 127u) == 0 && (((uintptr_t)table) & 15u) == 0) {
         const uint64_t nrows = (uint64_t)G * G;
@@ -921,7 +887,7 @@ hipError_t launch_marker_jobs(const ItemMap& m_blocks, const ItemMap& m_rows, co
                               uint32_t blocks, uint32_t row_units, void* mws, hipStream_t stream) {
     const MarkerWs ws{(unsigned*)mws, (unsigned long long*)((char*)mws + 256)};
     const ItemDev dummy{};
-    if (blocks) solid_marker_kernel<true><<<blocks, kMBlock, 0, stream>>>(dummy, m_blocks, ws, 1);
+    if (blocks) solid_marker_kernel<true, false><<<blocks, kMBlock, 0, stream>>>(dummy, m_blocks, ws, 1, RegionArgs{});
     if (row_units) suffix_jobs_kernel<<<row_units, 256, 0, stream>>>(m_rows);
     spill_kernel<<<kSpillBlocks, 256, 0, stream>>>(dummy, m_blocks, m_tris, ws);
     return hipGetLastError();

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "../../include/cuda_voxelizer_amd/vox_capi.h"
+#include "region.h"
 #include "slab.h"
 #include "tile.h"
 
@@ -50,6 +51,33 @@ bool slab_layout_fits(const SlabPlan& P, uint64_t n, const SlabLayout& L);
 hipError_t launch_slab(const vox_info& info, const float* tris, uint32_t* table, const SlabPlan& P, bool solid,
                        bool accumulate, unsigned long long* offrow, void* mem, const SlabLayout& L,
                        hipStream_t stream);
+
+// Region algorithm (region.hip): phase 1 stages (word, mask) pairs per output
+// region (surface: chunk.hip, solid: marker.hip), the build pass writes each
+// region once.  K: capacity overrides (flag bits), -1 fields are built-in.
+bool make_region_plan(uint32_t G, int layout, bool solid, RegionPlan& P, const RegionKnobs& K);
+bool region_supported(uint32_t G, int layout, bool solid);
+// workgroups: phase-1 workgroups of the call (each stages into its own slice)
+hipError_t region_workspace(const RegionPlan& P, uint64_t workgroups, hipStream_t stream, RegionWs& W);
+hipError_t launch_region_build(const RegionPlan& P, const RegionWs& W, uint32_t* table, bool accumulate,
+                               hipStream_t stream);
+hipError_t launch_surface_region_chunks(const vox_info& info, const float* tris, int layout, const RegionArgs& RA,
+                                        hipStream_t stream, uint32_t big, int waves);
+hipError_t launch_surface_region(const vox_info& info, const float* tris, uint32_t* table, int layout,
+                                 bool accumulate, hipStream_t stream, uint32_t big = 0, int waves = 0,
+                                 const RegionKnobs& K = RegionKnobs{});
+hipError_t launch_solid_region(const vox_info& info, const float* tris, uint32_t* table, int layout,
+                               bool accumulate, unsigned long long* offrow, hipStream_t stream, uint32_t big = 0,
+                               const RegionKnobs& K = RegionKnobs{});
+// VOX_REGION_* bits of the flags
+inline RegionKnobs region_knobs(unsigned flags) {
+    RegionKnobs K;
+    const unsigned w = (flags >> 8) & 15u, st = (flags >> 12) & 31u, d = (flags >> 17) & 15u;
+    if (w) K.lg_words = (int)w + 9;
+    if (st) K.lg_stage = (int)st - 1;
+    if (d) K.lg_desc = (int)d - 1;
+    return K;
+}
 
 bool chunk_supported(uint32_t G);
 // Default algorithm: a mesh with few triangles for its grid has large voxel

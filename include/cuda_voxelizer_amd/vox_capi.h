@@ -82,6 +82,20 @@ typedef enum vox_status {
                                     once -- no zero-fill pass, no global atomics
                                     except for rare wide triangles and runs that
                                     leave their row */
+#define VOX_FLAG_ALGO_REGION 0x40u /* power-of-two G in 32..2048 (otherwise the default):
+                                    each workgroup merges its chunk's voxels (surface)
+                                    or run-end markers (solid) in LDS and stages them
+                                    as (word, mask) pairs per output region; then one
+                                    workgroup per region merges its pairs in LDS (solid:
+                                    and takes the row suffix XOR) and writes every word
+                                    of the region once -- no zero-fill pass, and global
+                                    atomics only for the staging reservations */
+/* Region capacity bits (with VOX_FLAG_ALGO_REGION; a zero field is the library's
+   choice).  Every setting gives the same table: pairs that find no room take a
+   scratch-table fallback, so small capacities only cost time. */
+#define VOX_REGION_WORDS(lg) ((((unsigned)(lg) - 9u) & 15u) << 8)  /* 2^lg words per region, 10..13 */
+#define VOX_REGION_STAGE(lg) ((((unsigned)(lg) + 1u) & 31u) << 12) /* 2^lg staged pairs per chunk, 0..10 */
+#define VOX_REGION_DESC(lg) ((((unsigned)(lg) + 1u) & 15u) << 17)  /* 2^lg descriptors per region, 0..9 */
 /* Slab tuning bits (with VOX_FLAG_ALGO_SLAB; a zero field is the library's
    choice).  Every setting gives the same table. */
 #define VOX_SLAB_TY(lg) ((((unsigned)(lg) + 1u) & 15u) << 8)     /* 2^lg rows in y per tile */

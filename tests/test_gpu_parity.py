@@ -14,8 +14,8 @@ from tests.conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
-ALGOS = ["auto", "tile", "chunk", "direct", "slab"]
-SOLID_ALGOS = ["auto", "marker", "tile", "direct", "slab"]
+ALGOS = ["auto", "tile", "chunk", "direct", "slab", "region"]
+SOLID_ALGOS = ["auto", "marker", "tile", "direct", "slab", "region"]
 
 
 def _dev_tris(tris, dev):
@@ -300,7 +300,7 @@ def test_solid_edges_through_column_centres(oracle, gpu, algo, morton):
     assert np.unpackbits(got.view(np.uint8)).any()
 
 
-@pytest.mark.parametrize("algo", ["marker", "tile"])
+@pytest.mark.parametrize("algo", ["marker", "tile", "region"])
 @pytest.mark.parametrize("grid,level", [(64, 1), (128, 2), (256, 5)])
 def test_solid_coarse_meshes_both_layouts(oracle, gpu, algo, grid, level):
     """Coarse meshes (large triangles: the marker algorithm's wide path) and a
@@ -312,7 +312,7 @@ def test_solid_coarse_meshes_both_layouts(oracle, gpu, algo, grid, level):
         assert _mismatch(got, exp) == 0, morton
 
 
-@pytest.mark.parametrize("algo", ["marker", "tile"])
+@pytest.mark.parametrize("algo", ["marker", "tile", "region"])
 def test_sh_sphere_512_solid_morton(oracle, gpu, algo):
     from cuda_voxelizer_amd import meshgen
     v, f = meshgen.icosphere_sh()

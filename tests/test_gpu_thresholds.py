@@ -9,7 +9,7 @@ from tests import kats
 
 pytestmark = pytest.mark.gpu
 
-SOLID_ALGOS = ["auto", "marker", "tile", "direct", "slab"]
+SOLID_ALGOS = ["auto", "marker", "tile", "direct", "slab", "region"]
 FAMILIES = {"normal": kats.normal_threshold_cases, "edge_t": kats.edge_threshold_cases,
             "edge_ties": kats.edge_tie_cases}
 
