@@ -98,8 +98,9 @@ __device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__
         word_or<REG>(S, table, ident, region_row_key(RA.P, (uint32_t)cx >> 5, (uint32_t)y, (uint32_t)z),
                      __brev(mask), RA);
     } else if ((G & 31u) == 0) {
-        const uint64_t w = ((uint64_t)cx + (uint64_t)y * G + (uint64_t)z * G * G) >> 5;
-        word_or<REG>(S, table, ident, (uint32_t)w, __brev(mask), RA);
+        // (z G + y) G/32 + cx/32 < G^3/32 < 2^32 (chunk_supported), so 32-bit arithmetic is exact
+        const uint32_t w = ((uint32_t)z * G + (uint32_t)y) * (G >> 5) + ((uint32_t)cx >> 5);
+        word_or<REG>(S, table, ident, w, __brev(mask), RA);
     } else {
         // linear rows not aligned to words: per voxel
         const uint64_t row = (uint64_t)y * G + (uint64_t)z * G * G;
