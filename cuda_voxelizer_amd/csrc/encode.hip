@@ -144,11 +144,16 @@ __device__ __forceinline__ uint32_t pairs_in(int64_t s, int64_t a, int64_t b) {
 }
 
 // The pairs that start in one word (positions p0 .. end-1, change mask B, run
-// start s in force at p0, updated to the word's last change).  EMIT: their
-// start positions and value bytes from offset o.  Returns their number.
+// start s in force at p0, updated to the word's last change).  Counting: their
+// number, and `first` (while negative) set to the first one's start.  EMIT:
+// each pair's value byte at out[2 o]; the count byte of the pair before it
+// (start `prev`, offset `prev_o`, while prev >= 0) at out[2 prev_o + 1] = its
+// distance to this one, and (prev, prev_o) moved to it; no byte is written at
+// or past pair `cap`.  Returns their number.
 template <bool EMIT>
 __device__ __forceinline__ uint32_t word_pairs(uint32_t bits, uint32_t B, int64_t p0, int64_t end, int64_t& s,
-                                               uint64_t o, uint64_t* __restrict__ starts, uint8_t* __restrict__ out) {
+                                               uint64_t o, int64_t& first, int64_t& prev, uint64_t& prev_o,
+                                               uint64_t cap, uint8_t* __restrict__ out) {
     int64_t a = p0;
     uint32_t count = 0;
     for (;;) {
@@ -157,13 +162,17 @@ __device__ __forceinline__ uint32_t word_pairs(uint32_t bits, uint32_t B, int64_
         if (b >= a) {
             if (EMIT) {
                 for (int64_t q = s + ((a - s + 254) / 255) * 255; q <= b; q += 255) {
-                    starts[o] = (uint64_t)q;
-                    out[2 * o] = (uint8_t)((bits >> (31 - (int)(q - p0))) & 1u);
+                    if (o < cap) out[2 * o] = (uint8_t)((bits >> (31 - (int)(q - p0))) & 1u);
+                    if (prev >= 0 && prev_o < cap) out[2 * prev_o + 1] = (uint8_t)(q - prev);
+                    prev = q;
+                    prev_o = o;
                     o++;
                     count++;
                 }
             } else {
-                count += pairs_in(s, a, b);
+                const uint32_t c = pairs_in(s, a, b);
+                if (c && first < 0) first = s + ((a - s + 254) / 255) * 255;
+                count += c;
             }
         }
         if (i >= 32) break;
@@ -221,71 +230,69 @@ __global__ void __launch_bounds__(256) rle_tile_last_kernel(const uint32_t* __re
 
 // Pass 3: with the run start in force at each tile (exclusive max-scan of
 // pass 1), the pairs of each tile; each thread keeps the run start in force at
-// its first word and its pair offset inside the tile for pass 5.
+// its first word, its pair offset inside the tile and the start of its first
+// pair for pass 5.  (Pairs start at most 255 positions apart, so every thread's
+// 256 positions hold one, except past the end of the table.)
 __global__ void __launch_bounds__(256) rle_tile_count_kernel(const uint32_t* __restrict__ P, uint64_t words, uint64_t n,
                                                              const int64_t* __restrict__ tile_s_in,
-                                                             uint32_t* __restrict__ tile_cnt,
+                                                             uint64_t* __restrict__ tile_cnt,
                                                              int64_t* __restrict__ thread_s,
-                                                             uint32_t* __restrict__ thread_off) {
+                                                             uint32_t* __restrict__ thread_off,
+                                                             int64_t* __restrict__ thread_first) {
     const uint64_t w0 = blockIdx.x * kRleTile + (uint64_t)threadIdx.x * kRlePer;
     RleWords R;
     rle_words(P, words, n, w0, R);
     uint64_t total;
     const int64_t ex = (int64_t)block_excl_scan<MaxOp>((uint64_t)R.last, &total);
     const int64_t s_in = tile_s_in[blockIdx.x] > ex ? tile_s_in[blockIdx.x] : ex;
-    int64_t s = s_in;
+    int64_t s = s_in, first = -1, prev = -1;
+    uint64_t prev_o = 0;
     uint32_t cnt = 0;
 #pragma unroll
     for (int u = 0; u < kRlePer; u++) {
         const uint64_t w = w0 + u;
         if (w >= words) break;
         const int64_t end = (int64_t)((n - w * 32) < 32 ? n : w * 32 + 32);
-        cnt += word_pairs<false>(R.bits[u], R.B[u], (int64_t)(w * 32), end, s, 0, nullptr, nullptr);
+        cnt += word_pairs<false>(R.bits[u], R.B[u], (int64_t)(w * 32), end, s, 0, first, prev, prev_o, 0, nullptr);
     }
     uint64_t tile_total;
     const uint64_t o0 = block_excl_scan<SumOp>(cnt, &tile_total);
     const uint64_t g = blockIdx.x * 256ull + threadIdx.x;
     thread_s[g] = s_in;
     thread_off[g] = (uint32_t)o0;
-    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = (uint32_t)tile_total;
+    thread_first[g] = first;
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = tile_total;
 }
 
-// Pass 5: each thread emits the pairs of its words from its tile's offset
-// (exclusive sum-scan of the counts) plus its offset inside the tile.
+// Pass 5: each thread emits the (value, count) pairs of its words from its
+// tile's offset (exclusive sum-scan of the counts) plus its offset inside the
+// tile; its last pair runs to the next thread's first pair (or to the end of
+// the table).  Pairs past `cap` are not written (the caller re-runs the pass
+// with a larger buffer).
 __global__ void __launch_bounds__(256) rle_tile_emit_kernel(const uint32_t* __restrict__ P, uint64_t words, uint64_t n,
                                                             const int64_t* __restrict__ thread_s,
                                                             const uint32_t* __restrict__ thread_off,
-                                                            const uint64_t* __restrict__ tile_off,
-                                                            uint64_t* __restrict__ starts, uint8_t* __restrict__ out) {
+                                                            const int64_t* __restrict__ thread_first,
+                                                            uint64_t nthreads, const uint64_t* __restrict__ tile_off,
+                                                            uint64_t cap, uint8_t* __restrict__ out) {
     const uint64_t w0 = blockIdx.x * kRleTile + (uint64_t)threadIdx.x * kRlePer;
     if (w0 >= words) return;
+    const uint64_t g = blockIdx.x * 256ull + threadIdx.x;
+    const uint64_t o0 = tile_off[blockIdx.x] + thread_off[g];
+    const int64_t nf = g + 1 < nthreads ? thread_first[g + 1] : -1;
+    if (o0 >= cap) return;
     RleWords R;
     rle_words(P, words, n, w0, R);
-    const uint64_t g = blockIdx.x * 256ull + threadIdx.x;
-    int64_t s = thread_s[g];
-    uint64_t o = tile_off[blockIdx.x] + thread_off[g];
+    int64_t s = thread_s[g], first = -1, prev = -1;
+    uint64_t o = o0, prev_o = 0;
 #pragma unroll
     for (int u = 0; u < kRlePer; u++) {
         const uint64_t w = w0 + u;
         if (w >= words) break;
         const int64_t end = (int64_t)((n - w * 32) < 32 ? n : w * 32 + 32);
-        o += word_pairs<true>(R.bits[u], R.B[u], (int64_t)(w * 32), end, s, o, starts, out);
+        o += word_pairs<true>(R.bits[u], R.B[u], (int64_t)(w * 32), end, s, o, first, prev, prev_o, cap, out);
     }
-}
-
-__global__ void __launch_bounds__(256) copy_u32_to_u64_kernel(const uint32_t* __restrict__ in,
-                                                              uint64_t* __restrict__ out, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        out[i] = in[i];
-}
-
-__global__ void __launch_bounds__(256) rle_count_bytes_kernel(const uint64_t* __restrict__ starts, uint64_t np,
-                                                              uint64_t n, uint8_t* __restrict__ out) {
-    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < np;
-         j += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t next = j + 1 < np ? starts[j + 1] : n;
-        out[2 * j + 1] = (uint8_t)(next - starts[j]);
-    }
+    if (prev >= 0 && prev_o < cap) out[2 * prev_o + 1] = (uint8_t)((nf >= 0 ? nf : (int64_t)n) - prev);
 }
 
 // ---------------------------------------------------------------------------
@@ -540,7 +547,6 @@ struct vox_encoder {
     void* off = nullptr;      uint64_t off_bytes = 0;     // uint64 per word
     void* aux = nullptr;      uint64_t aux_bytes = 0;     // int64 per word (run starts)
     void* sums = nullptr;     uint64_t sums_bytes = 0;    // scan block sums
-    void* starts = nullptr;   uint64_t starts_bytes = 0;  // uint64 per binvox pair
     void* out = nullptr;      uint64_t out_bytes = 0;     // encoded bytes
     uint64_t* host = nullptr;                             // pinned: 2 words read back
     uint64_t result = 0;
@@ -763,7 +769,7 @@ vox_status vox_encoder_create(void* stream, vox_encoder** out) {
 vox_status vox_encoder_destroy(vox_encoder* e) {
     if (!e) return VOX_OK;
     hipError_t he = hipStreamSynchronize(e->stream);
-    void* bufs[] = {e->lin, e->perm, e->cnt, e->off, e->aux, e->sums, e->starts, e->out};
+    void* bufs[] = {e->lin, e->perm, e->cnt, e->off, e->aux, e->sums, e->out};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (e->host) (void)hipHostFree(e->host);
@@ -825,29 +831,36 @@ vox_status vox_encode(vox_encoder* e, const uint32_t* d_table, uint32_t grid, in
     if (format == VOX_ENC_BINVOX) {
         // tiles of kRleTile words: last change per tile -> exclusive max-scan (one
         // workgroup) -> pairs per tile -> exclusive sum-scan -> pairs emitted
-        const uint64_t nt = (words + kRleTile - 1) / kRleTile;
-        ENC_CHECK(grow(e->aux, e->aux_bytes, nt * 8 + nt * 256 * 12 + 16));
+        const uint64_t nt = (words + kRleTile - 1) / kRleTile, nth = nt * 256;
+        ENC_CHECK(grow(e->aux, e->aux_bytes, nt * 8 + nth * 20 + 16));
         int64_t* tile_s = (int64_t*)e->aux;                            // last change, then run start, per tile
         int64_t* thread_s = tile_s + nt;                               // run start at each thread's words
-        uint32_t* thread_off = (uint32_t*)(thread_s + nt * 256);      // pair offset inside the tile
-        uint32_t* tile_cnt = cnt;                                      // pairs per tile
-        uint64_t* tile_off = off;                                      // their exclusive prefix
+        int64_t* thread_first = thread_s + nth;                        // start of each thread's first pair
+        uint32_t* thread_off = (uint32_t*)(thread_first + nth);       // pair offset inside the tile
+        uint64_t* tile_off = off;                                      // pairs per tile, then their prefix
         rle_tile_last_kernel<<<(unsigned)nt, 256, 0, st>>>(P, words, n, tile_s);
         scan_top_kernel<MaxOp><<<1, 256, 0, st>>>((uint64_t*)tile_s, nt);
-        rle_tile_count_kernel<<<(unsigned)nt, 256, 0, st>>>(P, words, n, tile_s, tile_cnt, thread_s, thread_off);
-        e->host[1] = 0;
-        ENC_HIP(hipMemcpyAsync(e->host + 1, tile_cnt + nt - 1, 4, hipMemcpyDeviceToHost, st), "read-back");
-        copy_u32_to_u64_kernel<<<grid_for(nt), 256, 0, st>>>(tile_cnt, tile_off, nt);
+        rle_tile_count_kernel<<<(unsigned)nt, 256, 0, st>>>(P, words, n, tile_s, tile_off, thread_s, thread_off,
+                                                            thread_first);
+        ENC_HIP(hipMemcpyAsync(e->host + 1, tile_off + nt - 1, 8, hipMemcpyDeviceToHost, st), "read-back");
         scan_top_kernel<SumOp><<<1, 256, 0, st>>>(tile_off, nt);
         ENC_HIP(hipMemcpyAsync(e->host, tile_off + nt - 1, 8, hipMemcpyDeviceToHost, st), "read-back");
-        ENC_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
-        const uint64_t np = e->host[0] + (e->host[1] & 0xFFFFFFFFull);
-        ENC_CHECK(grow(e->starts, e->starts_bytes, np * 8 + 8));
-        ENC_CHECK(grow(e->out, e->out_bytes, np * 2 + 16));
-        rle_tile_emit_kernel<<<(unsigned)nt, 256, 0, st>>>(P, words, n, thread_s, thread_off, tile_off,
-                                                           (uint64_t*)e->starts, (uint8_t*)e->out);
-        rle_count_bytes_kernel<<<grid_for(np), 256, 0, st>>>((const uint64_t*)e->starts, np, n, (uint8_t*)e->out);
+        // the pairs are emitted into the buffer as it is (a previous call's size, or a quarter
+        // pair per word) before their number is known; a larger result is emitted again
+        ENC_CHECK(grow(e->out, e->out_bytes, words / 2 + 16));
+        uint64_t cap = (e->out_bytes - 16) / 2;
+        rle_tile_emit_kernel<<<(unsigned)nt, 256, 0, st>>>(P, words, n, thread_s, thread_off, thread_first, nth,
+                                                           tile_off, cap, (uint8_t*)e->out);
         ENC_HIP(hipGetLastError(), "binvox kernels");
+        ENC_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
+        const uint64_t np = e->host[0] + e->host[1];
+        if (np > cap) {
+            ENC_CHECK(grow(e->out, e->out_bytes, np * 2 + 16));
+            cap = (e->out_bytes - 16) / 2;
+            rle_tile_emit_kernel<<<(unsigned)nt, 256, 0, st>>>(P, words, n, thread_s, thread_off, thread_first, nth,
+                                                               tile_off, cap, (uint8_t*)e->out);
+            ENC_HIP(hipGetLastError(), "binvox emit kernel");
+        }
         e->result = np * 2;
     } else {
         obj_kernel<false><<<gb, 256, 0, st>>>(P, words, G, format, cnt, nullptr, nullptr);

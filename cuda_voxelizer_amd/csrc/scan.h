@@ -63,16 +63,27 @@ __global__ void __launch_bounds__(256) scan_reduce_kernel(const T* __restrict__ 
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// One workgroup: exclusive scan of the block sums, in place.
+// One workgroup: exclusive scan of the block sums, in place; kScanPer
+// consecutive sums per thread, so up to kScanTile sums take one block scan.
 template <class Op>
 __global__ void __launch_bounds__(256) scan_top_kernel(uint64_t* __restrict__ sums, uint64_t nb) {
     uint64_t carry = Op::id();
-    for (uint64_t b0 = 0; b0 < nb; b0 += 256) {
-        const uint64_t i = b0 + threadIdx.x;
-        const uint64_t v = i < nb ? sums[i] : Op::id();
+    for (uint64_t b0 = 0; b0 < nb; b0 += kScanTile) {
+        const uint64_t i0 = b0 + (uint64_t)threadIdx.x * kScanPer;
+        uint64_t vals[kScanPer];
+        uint64_t v = Op::id();
+#pragma unroll
+        for (int k = 0; k < kScanPer; k++) {
+            vals[k] = i0 + k < nb ? sums[i0 + k] : Op::id();
+            v = Op::op(v, vals[k]);
+        }
         uint64_t total;
-        const uint64_t ex = block_excl_scan<Op>(v, &total);
-        if (i < nb) sums[i] = Op::op(carry, ex);
+        uint64_t run = Op::op(carry, block_excl_scan<Op>(v, &total));
+#pragma unroll
+        for (int k = 0; k < kScanPer; k++) {
+            if (i0 + k < nb) sums[i0 + k] = run;
+            run = Op::op(run, vals[k]);
+        }
         carry = Op::op(carry, total);
     }
 }

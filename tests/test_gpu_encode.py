@@ -73,3 +73,15 @@ def test_encoder_rejects_bad_arguments(gpu):
         io.encode(t, 4096, "binvox")
     with pytest.raises(VoxError):
         io.encode(t, 6, "binvox", morton=True)
+
+
+def test_binvox_encoder_reuse_grows(oracle, gpu):
+    """One encoder over tables whose pair counts rise and fall: the pairs are
+    emitted into the buffer a previous call left before their number is known,
+    and emitted again into a larger one when they do not fit."""
+    from cuda_voxelizer_amd import io
+    with io.Encoder() as enc:
+        for G, p in ((64, 0.001), (64, 0.5), (128, 0.4), (32, 0.0), (96, 0.02), (128, 0.45)):
+            table = pack(random_vox(G, G + 7, p))
+            print(f"encode binvox reuse G={G} p={p}", flush=True)
+            assert enc.encode(table, G, False, io.FORMATS["binvox"]) == oracle.binvox_body(table, G)
