@@ -400,7 +400,9 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* marke
                 for (int u = 0; u < 8; u++) {
                     const uint32_t i = i0 + 256u * (uint32_t)u;
                     const uint32_t zl = i / zq, k = i - zl * zq;
-                    if (i < total4) c4[((((uint64_t)(z0 + zl) * G + y0) * rw) >> 2) + k] = make_uint4(0u, 0u, 0u, 0u);
+                    // words already zero (rows without markers) need no store
+                    if (i < total4 && (v[u].x | v[u].y | v[u].z | v[u].w))
+                        c4[((((uint64_t)(z0 + zl) * G + y0) * rw) >> 2) + k] = make_uint4(0u, 0u, 0u, 0u);
                 }
             }
 #pragma unroll
@@ -444,7 +446,7 @@ __global__ void __launch_bounds__(256) marker_build_kernel(const uint32_t* marke
                 for (int u = 0; u < 8; u++) {
                     const uint32_t i = i0 + 256u * (uint32_t)u;
                     const uint32_t zl = i / (32u * rw), k = i % (32u * rw);
-                    if (i < total) c[((uint64_t)(z0 + zl) * G + y0) * rw + k] = 0u;
+                    if (i < total && v[u]) c[((uint64_t)(z0 + zl) * G + y0) * rw + k] = 0u;
                 }
             }
 #pragma unroll
