@@ -42,7 +42,8 @@ struct vox_batch {
     void* ws = nullptr;
     void* mws = nullptr;  // marker class: spill records and counters
     voxamd::WorkspaceLayout L{};
-    voxamd::JobClass fill, chunk, chunkw, marker, dsurf, dsolid, tsurf, tsolid;  // chunkw: 6 waves per SIMD
+    // chunk / chunkw (6 waves per SIMD): one class per layout specialisation (ChunkSpec)
+    voxamd::JobClass fill, chunk[3], chunkw[3], marker, dsurf, dsolid, tsurf, tsolid;
     int count = 0;
     uint64_t total_triangles = 0;
 };
@@ -59,12 +60,14 @@ static hipError_t launch_all(vox_batch* b, hipStream_t s) {
     const ItemDev dummy{};
     hipError_t e = hipSuccess;
     if (b->fill.total_a && (e = launch_fill_jobs(dummy, b->fill.m_a, b->fill.total_a, s)) != hipSuccess) return e;
-    if (b->chunk.total_a &&
-        (e = launch_chunk_jobs(dummy, b->chunk.m_a, b->chunk.total_a, b->chunk.loop, s, 5)) != hipSuccess)
-        return e;
-    if (b->chunkw.total_a &&
-        (e = launch_chunk_jobs(dummy, b->chunkw.m_a, b->chunkw.total_a, b->chunkw.loop, s, 6)) != hipSuccess)
-        return e;
+    for (int sp = 0; sp < 3; sp++) {
+        const JobClass& c5 = b->chunk[sp];
+        const JobClass& c6 = b->chunkw[sp];
+        if (c5.total_a && (e = launch_chunk_jobs(dummy, c5.m_a, c5.total_a, c5.loop, s, 5, sp)) != hipSuccess)
+            return e;
+        if (c6.total_a && (e = launch_chunk_jobs(dummy, c6.m_a, c6.total_a, c6.loop, s, 6, sp)) != hipSuccess)
+            return e;
+    }
     if (!b->marker.items.empty() &&
         (e = launch_marker_jobs(b->marker.m_a, b->marker.m_b, b->marker.m_c, b->marker.total_a, b->marker.total_b,
                                 b->mws, s)) != hipSuccess)
@@ -104,7 +107,8 @@ struct BatchPlan {
 };
 
 static void reset_classes(vox_batch* b) {
-    JobClass* classes[] = {&b->fill, &b->chunk, &b->chunkw, &b->marker, &b->dsurf, &b->dsolid, &b->tsurf, &b->tsolid};
+    JobClass* classes[] = {&b->fill,      &b->chunk[0],  &b->chunk[1], &b->chunk[2], &b->chunkw[0], &b->chunkw[1],
+                           &b->chunkw[2], &b->marker,   &b->dsurf,    &b->dsolid,   &b->tsurf,     &b->tsolid};
     for (JobClass* c : classes) *c = JobClass{};
     b->count = 0;
     b->total_triangles = 0;
@@ -165,9 +169,11 @@ static vox_status plan_batch(const vox_batch_item* items, int count, vox_batch* 
         } else if (a == CHUNK) {
             d.span = chunk_span(G, n);
             d.parts = chunk_parts(G, n);
-            // jobs that loop over chunks run in the 4-wave form: they stay in the 5-wave class
-            JobClass& c = (tune_waves(it.flags) == 6 && !(d.span > 1u && d.parts <= 1u)) ? b->chunkw : b->chunk;
-            if (d.span > 1u && d.parts <= 1u) c.loop = true;
+            // jobs that loop over chunks run in the 4-wave form (any layout): they stay in the 5-wave class
+            const bool loop = d.span > 1u && d.parts <= 1u;
+            const int sp = loop ? (int)kSpecAny : chunk_spec(it.layout, G);
+            JobClass& c = (tune_waves(it.flags) == 6 && !loop) ? b->chunkw[sp] : b->chunk[sp];
+            if (loop) c.loop = true;
             c.items.push_back(d);
             add_prefix(c.base_a, c.total_a, chunk_units(n, d.span, d.parts));
         } else if (a == DIRECT) {
@@ -191,7 +197,8 @@ static vox_status plan_batch(const vox_batch_item* items, int count, vox_batch* 
         }
     }
     // one device allocation: per class the job array and its prefix arrays
-    JobClass* classes[] = {&b->fill, &b->chunk, &b->chunkw, &b->marker, &b->dsurf, &b->dsolid, &b->tsurf, &b->tsolid};
+    JobClass* classes[] = {&b->fill,      &b->chunk[0],  &b->chunk[1], &b->chunk[2], &b->chunkw[0], &b->chunkw[1],
+                           &b->chunkw[2], &b->marker,   &b->dsurf,    &b->dsolid,   &b->tsurf,     &b->tsolid};
     size_t total = 0;
     auto reserve = [&](size_t bytes) {
         const size_t off = total;
@@ -224,7 +231,8 @@ static vox_status plan_batch(const vox_batch_item* items, int count, vox_batch* 
 // Point every class's job maps at a device copy of the packed image.
 static void bind_maps(vox_batch* b, void* dev) {
     char* dm = (char*)dev;
-    JobClass* classes[] = {&b->fill, &b->chunk, &b->chunkw, &b->marker, &b->dsurf, &b->dsolid, &b->tsurf, &b->tsolid};
+    JobClass* classes[] = {&b->fill,      &b->chunk[0],  &b->chunk[1], &b->chunk[2], &b->chunkw[0], &b->chunkw[1],
+                           &b->chunkw[2], &b->marker,   &b->dsurf,    &b->dsolid,   &b->tsurf,     &b->tsolid};
     for (JobClass* c : classes) {
         const ItemDev* it = (const ItemDev*)(dm + c->off_items);
         const int ni = (int)c->items.size();
@@ -315,7 +323,8 @@ vox_status vox_batch_info(const vox_batch* b, int64_t out[8]) {
     if (!b || !out) return fail(VOX_ERR_INVALID, "NULL argument");
     out[0] = b->count;
     out[1] = (int64_t)b->total_triangles;
-    out[2] = (int64_t)(b->chunk.items.size() + b->chunkw.items.size());
+    out[2] = 0;
+    for (int sp = 0; sp < 3; sp++) out[2] += (int64_t)(b->chunk[sp].items.size() + b->chunkw[sp].items.size());
     out[3] = (int64_t)b->tsurf.items.size();
     out[4] = (int64_t)b->tsolid.items.size();
     out[5] = (int64_t)(b->dsurf.items.size() + b->dsolid.items.size());

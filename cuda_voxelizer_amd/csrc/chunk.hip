@@ -80,15 +80,16 @@ __device__ __forceinline__ void word_or(ChunkShared& S, uint32_t* __restrict__ t
 
 // Passing voxels of row (y, z) as a mask (bit k = voxel cx + k, cx a multiple
 // of 32) -> hash, keyed by table word (REG, linear layout: by region row key).
-template <bool REG>
+// SPEC: what is known of every job at compile time (see ChunkSpec).
+template <bool REG, int SPEC>
 __device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__ table, bool ident,
                                           const vox_info& info, int layout, int cx, int y, int z, uint32_t mask,
                                           const RegionArgs& RA) {
     if (!mask) return;
     const uint32_t G = info.gridsize[0];
-    if (layout == VOX_LAYOUT_MORTON) {
+    if (SPEC == kSpecMorton || (SPEC == kSpecAny && layout == VOX_LAYOUT_MORTON)) {
         // codes of grids up to 1024 fit 30 bits: 32-bit spreads
-        const uint64_t base = G <= 1024u ? (uint64_t)(spread10((uint32_t)cx) | (spread10((uint32_t)y) << 1) |
+        const uint64_t base = (SPEC == kSpecMorton || G <= 1024u) ? (uint64_t)(spread10((uint32_t)cx) | (spread10((uint32_t)y) << 1) |
                                                       (spread10((uint32_t)z) << 2))
                                          : morton3((uint32_t)cx, (uint32_t)y, (uint32_t)z);
         const uint32_t yz_low = ((uint32_t)(y & 1) << 1) | ((uint32_t)(z & 1) << 2) | ((uint32_t)((y >> 1) & 1) << 4);
@@ -97,7 +98,7 @@ __device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__
     } else if (REG) {
         word_or<REG>(S, table, ident, region_row_key(RA.P, (uint32_t)cx >> 5, (uint32_t)y, (uint32_t)z),
                      __brev(mask), RA);
-    } else if ((G & 31u) == 0) {
+    } else if (SPEC == kSpecLinear || (G & 31u) == 0) {
         // (z G + y) G/32 + cx/32 < G^3/32 < 2^32 (chunk_supported), so 32-bit arithmetic is exact
         const uint32_t w = ((uint32_t)z * G + (uint32_t)y) * (G >> 5) + ((uint32_t)cx >> 5);
         word_or<REG>(S, table, ident, w, __brev(mask), RA);
@@ -126,7 +127,7 @@ __device__ __forceinline__ void emit_mask(ChunkShared& S, uint32_t* __restrict__
 // a row then tests the XY edges and the plane only at the x that passed ZX.
 // Every test is the same binary32 expression as in vox_device.h, evaluated
 // for a subset of the voxels: the resulting bits are unchanged.
-template <bool REG>
+template <bool REG, int SPEC>
 __device__ __forceinline__ void lane_raster(ChunkShared& S, uint32_t* __restrict__ table, bool ident,
                                             const SurfTri& T, const vox_info& info, int layout,
                                             const RegionArgs& RA) {
@@ -135,7 +136,8 @@ __device__ __forceinline__ void lane_raster(ChunkShared& S, uint32_t* __restrict
     const int c0 = xa & ~31, sh = xa - c0;
     // Morton codes of grids up to 1024 fit 30 bits: the x, y and z spreads are
     // computed once per lane, row and slice instead of once per emitted mask
-    const bool m32 = layout == VOX_LAYOUT_MORTON && info.gridsize[0] <= 1024u;
+    const bool m32 = SPEC == kSpecMorton ||
+                     (SPEC == kSpecAny && layout == VOX_LAYOUT_MORTON && info.gridsize[0] <= 1024u);
     const uint32_t sc0 = m32 ? spread10((uint32_t)c0) : 0u, sc1 = m32 ? spread10((uint32_t)c0 + 32u) : 0u;
     const auto put = [&](uint64_t w, uint32_t bits) { word_or<REG>(S, table, ident, (uint32_t)w, bits, RA); };
     for (int z = T.lo[2]; z <= T.hi[2]; z++) {
@@ -194,8 +196,8 @@ __device__ __forceinline__ void lane_raster(ChunkShared& S, uint32_t* __restrict
                 if (M >> 32) morton_chunk_words(sc1 | syz, yz_low, (uint32_t)(M >> 32), put);
                 continue;
             }
-            emit_mask<REG>(S, table, ident, info, layout, c0, y, z, (uint32_t)M, RA);
-            if (M >> 32) emit_mask<REG>(S, table, ident, info, layout, c0 + 32, y, z, (uint32_t)(M >> 32), RA);
+            emit_mask<REG, SPEC>(S, table, ident, info, layout, c0, y, z, (uint32_t)M, RA);
+            if (M >> 32) emit_mask<REG, SPEC>(S, table, ident, info, layout, c0 + 32, y, z, (uint32_t)(M >> 32), RA);
         }
     }
 }
@@ -221,8 +223,10 @@ __device__ __forceinline__ void staged_vertices(const ChunkShared& S, int lt, co
 // Workgroups per CU: the LDS allows six.  The bound holds the registers at
 // five waves per SIMD, or six with WIDE (a few spilled registers: it pays for
 // the linear layout, whose chunks touch more words and so wait longer on the
-// LDS table, not for the Morton layout), or four for the chunk loop.
-template <bool MULTI, bool LOOP, bool WIDE, bool REG>
+// LDS table, not for the Morton layout), or four for the chunk loop.  SPEC:
+// the layout when it is known at compile time, so the other layout's word
+// paths are compiled out (at six waves this removes the spills).
+template <bool MULTI, bool LOOP, bool WIDE, bool REG, int SPEC = kSpecAny>
 __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5))
     surface_chunk_kernel(ItemDev one, ItemMap M, RegionArgs RA) {
     __shared__ ChunkShared S;
@@ -236,7 +240,8 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5))
     const ItemDev& it = MULTI ? J.it : one;
     const vox_info& info = it.info;
     uint32_t* __restrict__ table = it.table;
-    const int layout = it.layout;
+    const int layout = SPEC == kSpecLinear ? (int)VOX_LAYOUT_LINEAR
+                       : SPEC == kSpecMorton ? (int)VOX_LAYOUT_MORTON : it.layout;
     const uint64_t n = info.n_triangles;
     const uint32_t parts = it.parts > 1u ? it.parts : 1u;
     const uint32_t span = (!LOOP || parts > 1u) ? 1u : (it.span > 1u ? it.span : 1u);
@@ -275,7 +280,7 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5))
             float v[3][3];
             staged_vertices(S, threadIdx.x, info, v);
             surface_setup_tests(v, info, T);  // the box is T's already
-            lane_raster<REG>(S, table, ident, T, info, layout, RA);  // nx <= 32 here
+            lane_raster<REG, SPEC>(S, table, ident, T, info, layout, RA);  // nx <= 32 here
         }
         __syncthreads();  // S.tri is dead from here on; S.wq is complete
         int myq = -1, nq = 0;  // this lane's queue slot (lane order), queue length
@@ -349,7 +354,7 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5))
                 const int cx = ((bx0 >> 5) + k) << 5;
                 if (cx > bx1) continue;
                 const int x0 = cx > bx0 ? cx : bx0, x1 = (cx + 31) < bx1 ? (cx + 31) : bx1;
-                emit_mask<REG>(S, table, ident, info, layout, cx, y, z,
+                emit_mask<REG, SPEC>(S, table, ident, info, layout, cx, y, z,
                                row_mask(R.n, R.d1, R.d2, R.xy, R.zx, ux, py, pz, x0, x1, cx), RA);
             }
             __syncthreads();  // the records are rewritten by the next round
@@ -441,26 +446,46 @@ hipError_t launch_fill_jobs(const ItemDev& one, const ItemMap& M, uint32_t units
     return hipGetLastError();
 }
 
+int chunk_spec(int layout, uint32_t G) {
+    if (layout == VOX_LAYOUT_LINEAR && (G & 31u) == 0) return kSpecLinear;
+    if (layout == VOX_LAYOUT_MORTON && G <= 1024u) return kSpecMorton;
+    return kSpecAny;
+}
+
+template <bool MULTI, bool WIDE>
+static void launch_chunk_spec(int spec, uint32_t units, hipStream_t stream, const ItemDev& one, const ItemMap& M) {
+    const RegionArgs none{};
+    if (spec == kSpecLinear)
+        surface_chunk_kernel<MULTI, false, WIDE, false, kSpecLinear><<<units, kChunk, 0, stream>>>(one, M, none);
+    else if (spec == kSpecMorton)
+        surface_chunk_kernel<MULTI, false, WIDE, false, kSpecMorton><<<units, kChunk, 0, stream>>>(one, M, none);
+    else
+        surface_chunk_kernel<MULTI, false, WIDE, false, kSpecAny><<<units, kChunk, 0, stream>>>(one, M, none);
+}
+
 // waves: 5 or 6 waves per SIMD for the one-chunk form (0: 6 for the linear
-// layout, 5 for Morton -- the measured better choice for each); the chunk
-// loop always runs at 4.
+// layout, 5 for Morton -- the measured better choice for each; a layout-
+// specialised form needs few enough registers to run at six either way); the
+// chunk loop always runs at 4.  spec: the layout every job of M shares
+// (chunk_spec of each, or kSpecAny); a single job's comes from `one`.
 hipError_t launch_chunk_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, bool loop, hipStream_t stream,
-                             int waves) {
+                             int waves, int spec) {
     if (units == 0) return hipSuccess;
     const bool wide = waves == 6 || (waves == 0 && !M.items && one.layout == VOX_LAYOUT_LINEAR);
+    if (!M.items) spec = chunk_spec(one.layout, one.info.gridsize[0]);
     const RegionArgs none{};
-    if (M.items && loop)
+    if (loop && M.items)
         surface_chunk_kernel<true, true, false, false><<<units, kChunk, 0, stream>>>(one, M, none);
-    else if (M.items && wide)
-        surface_chunk_kernel<true, false, true, false><<<units, kChunk, 0, stream>>>(one, M, none);
-    else if (M.items)
-        surface_chunk_kernel<true, false, false, false><<<units, kChunk, 0, stream>>>(one, M, none);
     else if (loop)
         surface_chunk_kernel<false, true, false, false><<<units, kChunk, 0, stream>>>(one, M, none);
+    else if (M.items && wide)
+        launch_chunk_spec<true, true>(spec, units, stream, one, M);
+    else if (M.items)
+        launch_chunk_spec<true, false>(spec, units, stream, one, M);
     else if (wide)
-        surface_chunk_kernel<false, false, true, false><<<units, kChunk, 0, stream>>>(one, M, none);
+        launch_chunk_spec<false, true>(spec, units, stream, one, M);
     else
-        surface_chunk_kernel<false, false, false, false><<<units, kChunk, 0, stream>>>(one, M, none);
+        launch_chunk_spec<false, false>(spec, units, stream, one, M);
     return hipGetLastError();
 }
 

@@ -97,8 +97,13 @@ uint32_t chunk_span(uint32_t G, uint64_t n_triangles);
 uint32_t chunk_parts(uint32_t G, uint64_t n_triangles);
 uint32_t chunk_units(uint64_t n_triangles, uint32_t span, uint32_t parts);
 hipError_t launch_fill_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, hipStream_t stream);
+// What a chunk kernel instantiation knows of its jobs' layout: nothing, linear
+// with G a multiple of 32 (whole words per row), or Morton with G <= 1024
+// (codes fit 30 bits).
+enum ChunkSpec : int { kSpecAny = 0, kSpecLinear = 1, kSpecMorton = 2 };
+int chunk_spec(int layout, uint32_t G);
 hipError_t launch_chunk_jobs(const ItemDev& one, const ItemMap& M, uint32_t units, bool loop, hipStream_t stream,
-                             int waves = 0);
+                             int waves = 0, int spec = kSpecAny);
 hipError_t launch_surface_chunk(const vox_info& info, const float* tris, uint32_t* table, int layout,
                                 hipStream_t stream, uint32_t big = 0, int waves = 0);
 // Tuning bits of the flags (VOX_TUNE_*): queue threshold, chunk waves per SIMD, tile size.
