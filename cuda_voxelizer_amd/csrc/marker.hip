@@ -292,19 +292,32 @@ __device__ __forceinline__ void suffix_rows_range(uint32_t* __restrict__ table, 
 // The same for G % 128 == 0 and a 16-byte aligned table: a lane holds 4
 // consecutive words (one 16-byte load), a row is rw/4 <= 64 lanes, a wave
 // holds 256/rw rows.
+// Rows as 16-byte groups, one lane per group, whole rows per wave: G a
+// multiple of 128 and at most 64 * 128, the table 16-byte aligned.
+__host__ __device__ inline bool rows4_ok(uint32_t G, const uint32_t* table) {
+    return (G & 127u) == 0 && G <= 8192u && (((uintptr_t)table) & 15u) == 0;
+}
+
+// CLEAR: the markers are read from `src` (the library's scratch marker table)
+// and the rows written to `table` whole; src words that held markers are
+// returned to zero, so the scratch table needs no fill before the next call.
+template <bool CLEAR = false>
 __device__ __forceinline__ void suffix_rows4_range(uint32_t* __restrict__ table, uint32_t G, uint64_t r_begin,
-                                                   uint64_t r_end, uint64_t wave, uint64_t nwaves) {
+                                                   uint64_t r_end, uint64_t wave, uint64_t nwaves,
+                                                   uint32_t* __restrict__ src = nullptr) {
     const uint32_t lpr = G >> 7;  // lanes per row
     const uint32_t per = 64 / lpr;
     const int lane = __lane_id();
     const uint32_t r_in = (uint32_t)lane / lpr, j = (uint32_t)lane % lpr;
     const bool active_lane = r_in < per;
     uint4* t4 = (uint4*)table;
+    uint4* s4 = CLEAR ? (uint4*)src : t4;
     for (uint64_t r0 = r_begin + wave * per; r0 < r_end; r0 += nwaves * per) {
         const uint64_t r = r0 + r_in;
         const bool act = active_lane && r < r_end;
         const uint64_t idx = r * lpr + j;
-        uint4 v = act ? t4[idx] : make_uint4(0u, 0u, 0u, 0u);
+        uint4 v = act ? s4[idx] : make_uint4(0u, 0u, 0u, 0u);
+        if (CLEAR && (v.x | v.y | v.z | v.w)) s4[idx] = make_uint4(0u, 0u, 0u, 0u);
         const uint32_t pa = __popc(v.x) & 1, pb = __popc(v.y) & 1, pc = __popc(v.z) & 1, pd = __popc(v.w) & 1;
         const unsigned long long par = __ballot(act && (pa ^ pb ^ pc ^ pd));
         const uint32_t seg0 = r_in * lpr;
@@ -334,6 +347,16 @@ __global__ void __launch_bounds__(256) suffix_rows4_kernel(uint32_t* __restrict_
     suffix_rows4_range(table, G, 0, nrows, wave, nwaves);
 }
 
+// Markers in the scratch table -> the rows of the linear table (every word
+// written), the scratch returned to zero.
+__global__ void __launch_bounds__(256) suffix_scratch_rows4_kernel(uint32_t* __restrict__ markers,
+                                                                   uint32_t* __restrict__ table, uint32_t G,
+                                                                   uint64_t nrows) {
+    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    suffix_rows4_range<true>(table, G, 0, nrows, wave, nwaves, markers);
+}
+
 // Batched form: work unit = kSuffixRows rows of one job's (linear, in-place)
 // table.
 __global__ void __launch_bounds__(256) suffix_jobs_kernel(ItemMap M) {
@@ -347,7 +370,7 @@ __global__ void __launch_bounds__(256) suffix_jobs_kernel(ItemMap M) {
     const uint64_t r0 = (uint64_t)J.local * kSuffixRows;
     const uint64_t r1 = r0 + kSuffixRows < nrows ? r0 + kSuffixRows : nrows;
     const uint64_t wave = threadIdx.x >> 6;
-    if ((G & 127u) == 0 && (((uintptr_t)table) & 15u) == 0)
+    if (rows4_ok(G, table))
         suffix_rows4_range(table, G, r0, r1, wave, 4);
     else
         suffix_rows_range(table, G, r0, r1, wave, 4);
@@ -659,7 +682,12 @@ hipError_t launch_marker_in_place(const vox_info& info, const float* tris, uint3
 hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t* table, int layout,
                                bool accumulate, unsigned long long* offrow, hipStream_t stream, uint32_t big) {
     const uint32_t G = info.gridsize[0];
-    const bool in_place = layout == VOX_LAYOUT_LINEAR && !accumulate;
+    // linear, no accumulation: markers into the scratch table and one streaming
+    // pass (suffix into the table, scratch back to zero) where rows are whole
+    // 16-byte groups and the scratch table can be had; in place (zero-fill,
+    // markers, suffix) otherwise
+    bool stream_rows = layout == VOX_LAYOUT_LINEAR && !accumulate && rows4_ok(G, table);
+    bool in_place = layout == VOX_LAYOUT_LINEAR && !accumulate && !stream_rows;
     const uint64_t table_bytes = (uint64_t)G * G * G / 8;
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
@@ -684,9 +712,17 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
                 if (e != hipSuccess) return e;
             }
             e = hipMalloc(&b.scratch, table_bytes);
-            if (e != hipSuccess) return e;
-            b.scratch_bytes = table_bytes;
-            b.scratch_dirty = true;
+            if (e == hipErrorOutOfMemory && stream_rows) {
+                (void)hipGetLastError();  // the in-place form needs no scratch table
+                b.scratch = nullptr;
+                stream_rows = false;
+                in_place = true;
+            } else if (e != hipSuccess) {
+                return e;
+            } else {
+                b.scratch_bytes = table_bytes;
+                b.scratch_dirty = true;
+            }
         }
         ws = MarkerWs{(unsigned*)b.ws, (unsigned long long*)((char*)b.ws + 256)};
         if (!in_place) {
@@ -717,10 +753,18 @@ hipError_t launch_solid_marker(const vox_info& info, const float* tris, uint32_t
         solid_marker_kernel<false, false><<<(unsigned)blocks, kMBlock, 0, stream>>>(one, none, ws, linear ? 1 : 0,
                                                                                     RegionArgs{});
     }
-    const uint32_t rz = linear ? 1u : (G <= 1024u ? 8u : 2u);
-    const uint32_t nblocks = (G / 32u) * (G / rz);
-    marker_build_kernel<true><<<nblocks, 256, 4ull * 32u * rz * (G / 32u + 1u), stream>>>(markers, table, G, rz, layout,
-                                                                             accumulate ? 1 : 0);
+    if (stream_rows) {
+        const uint64_t nrows = (uint64_t)G * G;
+        const uint64_t per = 64 / (G >> 7);
+        uint64_t blocks = (nrows / per + 3) / 4;
+        if (blocks > 262144) blocks = 262144;
+        suffix_scratch_rows4_kernel<<<(unsigned)blocks, 256, 0, stream>>>(markers, table, G, nrows);
+    } else {
+        const uint32_t rz = linear ? 1u : (G <= 1024u ? 8u : 2u);
+        const uint32_t nblocks = (G / 32u) * (G / rz);
+        marker_build_kernel<true><<<nblocks, 256, 4ull * 32u * rz * (G / 32u + 1u), stream>>>(
+            markers, table, G, rz, layout, accumulate ? 1 : 0);
+    }
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     {
@@ -804,7 +848,7 @@ hipError_t launch_marker_in_place(const vox_info& info, const float* tris, uint3
         const uint64_t blocks = (info.n_triangles + kMBlock - 1) / kMBlock;
         solid_marker_kernel<false, false><<<(unsigned)blocks, kMBlock, 0, stream>>>(one, none, ws, 1, RegionArgs{});
     }
-    if ((G & 127u) == 0 && (((uintptr_t)table) & 15u) == 0) {
+    if (rows4_ok(G, table)) {
         const uint64_t nrows = (uint64_t)G * G;
         const uint64_t per = 64 / (G >> 7);
         uint64_t blocks = (nrows / per + 3) / 4;
