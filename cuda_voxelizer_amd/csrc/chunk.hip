@@ -61,12 +61,11 @@ __device__ __forceinline__ void word_or(ChunkShared& S, uint32_t* __restrict__ t
     uint32_t h = ident ? (key & (kHashCap - 1)) : (key * 2654435761u) >> (32 - kHashBits);
 #pragma unroll 1
     for (int probe = 0; probe < (REG ? kRegionProbes : kProbes); probe++) {
-        uint32_t k = S.keys[h];
-        if (k == kEmpty) {
-            k = atomicCAS(&S.keys[h], kEmpty, key);
-            if (k == kEmpty) k = key;
-        }
-        if (k == key) {
+        // one round trip: the CAS claims an empty slot or returns the key
+        // already there (a read first, then a CAS on empty slots, took two for
+        // every new word: 1024^3 linear torus 114 vs 108 us per call)
+        const uint32_t k = atomicCAS(&S.keys[h], kEmpty, key);
+        if (k == kEmpty || k == key) {
             atomicOr(&S.vals[h], mask);
             return;
         }

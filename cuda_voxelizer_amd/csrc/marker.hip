@@ -75,12 +75,8 @@ __device__ __forceinline__ void marker_xor(MarkerLds<true>& H, uint32_t key, uin
     uint32_t h = (key * 2654435761u) >> (32 - kMHashBits);
 #pragma unroll 1
     for (int probe = 0; probe < kRegionProbes; probe++) {
-        uint32_t k = H.keys[h];
-        if (k == kRegionEmpty) {
-            k = atomicCAS(&H.keys[h], kRegionEmpty, key);
-            if (k == kRegionEmpty) k = key;
-        }
-        if (k == key) {
+        const uint32_t k = atomicCAS(&H.keys[h], kRegionEmpty, key);  // claims or finds, in one round trip
+        if (k == kRegionEmpty || k == key) {
             atomicXor(&H.vals[h], mask);
             return;
         }

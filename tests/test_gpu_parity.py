@@ -342,6 +342,40 @@ def test_solid_marker_scratch_reuse(oracle, gpu):
         assert _mismatch(_host_table(t), exp) == 0, (grid, morton, acc)
 
 
+@pytest.mark.parametrize("solid", [False, True])
+@pytest.mark.parametrize("morton", [False, True])
+def test_table_not_16_byte_aligned(oracle, gpu, solid, morton):
+    """A table that starts 4 bytes past a 16-byte boundary (a C caller's
+    pointer may be only word-aligned): every algorithm's 16-byte paths (fill,
+    streaming solid suffix, brick stores, slab and region write-outs) must
+    step aside, and the words on either side stay as they were."""
+    import torch
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import meshgen
+    grid = 128
+    v, f = meshgen.icosphere_sh(level=4, seed=5)
+    tris = meshgen.to_aos(v, f)
+    info = V.voxinfo_for_mesh(tris.reshape(-1, 3), grid, tris.shape[0])
+    d = _dev_tris(tris, gpu)
+    words = V.table_words(grid)
+    if solid:
+        exp, _, _ = oracle.solid(_oracle_info(oracle, info), tris, morton=morton)
+    else:
+        exp, _ = oracle.surface(_oracle_info(oracle, info), tris, morton=morton)
+    for algo in (SOLID_ALGOS if solid else ALGOS):
+        print(f"unaligned table solid={solid} morton={morton} algo={algo}", flush=True)
+        buf = torch.full((words + 8,), -0x55555556, dtype=torch.int32, device=gpu)  # 0xAAAAAAAA guards
+        t = buf[1:1 + words]
+        assert t.data_ptr() % 16 == 4
+        if solid:
+            V.voxelize_solid_table(info, d, morton=morton, table=t, algo=algo)
+        else:
+            V.voxelize_surface(info, d, morton=morton, table=t, algo=algo)
+        host = buf.cpu().numpy().view(np.uint32)
+        assert _mismatch(host[1:1 + words], exp) == 0, algo
+        assert host[0] == 0xAAAAAAAA and (host[1 + words:] == 0xAAAAAAAA).all(), algo
+
+
 @pytest.mark.parametrize("grid", [64, 96])
 def test_solid_deep_columns(oracle, gpu, grid):
     """Forty nested closed shells: up to 80 crossings per column, so most
