@@ -2,19 +2,18 @@
 //
 // One workgroup takes a chunk of 256 consecutive triangles (meshes are stored
 // with spatially coherent triangle order, so a chunk covers a compact
-// region).  The chunk is staged through LDS with coalesced loads; each lane
-// rasterizes its own triangle (ZX edges once per z slice, then per row the YZ
-// edges and, at the x that passed ZX, the plane and XY edges) and ORs the passing
-// bits into an LDS hash table keyed by table word, so bits that several
-// triangles of the chunk set in one word are merged before memory is
-// touched.  Triangles with a large voxel box are queued; afterwards their
-// owners publish test records to LDS (up to kRecs per round) and the
-// workgroup deals the rounds' (row, 32-voxel x chunk) items evenly over its
-// lanes, testing only the x range that the plane band leaves open.  The
-// table is then updated with one
-// non-returning global atomicOr per distinct word; words that do not fit in
-// the hash go straight to global atomics.  The table is zero-filled
-// beforehand (unless accumulating).
+// region).  Each lane loads and rasterizes its own triangle (ZX edges once
+// per z slice, then per row the YZ edges and, at the x that passed ZX, the
+// plane and XY edges) and ORs the passing bits into an LDS hash table keyed
+// by table word, so bits that several triangles of the chunk set in one word
+// are merged before memory is touched.  Triangles with a large voxel box are
+// queued; afterwards their owners publish test records to LDS (up to kRecs
+// per round) and the workgroup deals the rounds' (row, 32-voxel x chunk)
+// items evenly over its lanes, testing only the x range that the plane band
+// leaves open.  The table is then updated with one non-returning global
+// atomicOr per distinct word; words that do not fit in the hash go straight
+// to global atomics.  The table is zero-filled beforehand (unless
+// accumulating).
 #include "jobs.h"
 #include "raster.h"
 #include "region.h"
@@ -39,14 +38,12 @@ struct ChunkShared {
     uint32_t keys[kHashCap];
     uint32_t vals[kHashCap];
     union {
-        float tri[kChunk * 9];  // staged vertices (until every lane has its setup)
-        SurfRec recs[kRecs];    // then: records of the queued triangles of one round
+        SurfRec recs[kRecs];      // records of the queued triangles of one round
         RegionFlushShared flush;  // last: grouping of the region algorithm's pairs
     };
     unsigned long long incl[kRecs];  // inclusive prefix of the round's item counts
     unsigned wq[kChunk / 64];        // queued triangles per wave
 };
-static_assert(sizeof(SurfRec) * kRecs <= sizeof(float) * kChunk * 9, "records must fit the staging area");
 
 // ident: the slot is the key modulo kHashCap.  Used when the whole table has
 // at most kHashCap words (its slots are preset to their keys, so every word is
@@ -201,15 +198,6 @@ __device__ __forceinline__ void lane_raster(ChunkShared& S, uint32_t* __restrict
     }
 }
 
-__device__ __forceinline__ void staged_vertices(const ChunkShared& S, int lt, const vox_info& info, float v[3][3]) {
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        v[k][0] = S.tri[lt * 9 + 3 * k + 0] - info.bbox_min[0];
-        v[k][1] = S.tri[lt * 9 + 3 * k + 1] - info.bbox_min[1];
-        v[k][2] = S.tri[lt * 9 + 3 * k + 2] - info.bbox_min[2];
-    }
-}
-
 // Work unit: `span` consecutive chunks of kChunk triangles of one job, merged
 // in one LDS table and flushed once -- or, with `parts` > 1, one chunk whose
 // queued items are shared by `parts` workgroups (each does the setup, takes
@@ -259,14 +247,15 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5))
     uint64_t first = begin;
     do {
         const int count = (int)(end - first < (uint64_t)kChunk ? end - first : (uint64_t)kChunk);
-        const float* src = it.tris + first * 9;
-        stage_floats<kChunk>(S.tri, src, count * 9);
-        __syncthreads();
+        // each lane loads its own triangle: the nine loads of a lane are in
+        // flight together and a wave's 2304 bytes are contiguous, so they are
+        // served by the cache after the first; staging the chunk through LDS
+        // cost a barrier and 9 KB of LDS (512^3 Morton torus 51.2 vs 48.0 us)
         bool big = false;
         SurfTri T;
+        float v[3][3];
         if ((int)threadIdx.x < count) {
-            float v[3][3];
-            staged_vertices(S, threadIdx.x, info, v);
+            load_tri(it.tris, first + threadIdx.x, info, v);
             surface_box(v, info, T.lo, T.hi);
             const int nx = T.hi[0] - T.lo[0] + 1;
             const int64_t vol = (int64_t)nx * (T.hi[1] - T.lo[1] + 1) * (T.hi[2] - T.lo[2] + 1);
@@ -276,12 +265,10 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5))
         const int wave = (int)(threadIdx.x >> 6);
         if (__lane_id() == 0) S.wq[wave] = (unsigned)__popcll(bal);
         if (part == 0 && (int)threadIdx.x < count && !big) {
-            float v[3][3];
-            staged_vertices(S, threadIdx.x, info, v);
             surface_setup_tests(v, info, T);  // the box is T's already
             lane_raster<REG, SPEC>(S, table, ident, T, info, layout, RA);  // nx <= 32 here
         }
-        __syncthreads();  // S.tri is dead from here on; S.wq is complete
+        __syncthreads();  // S.wq is complete
         int myq = -1, nq = 0;  // this lane's queue slot (lane order), queue length
 #pragma unroll
         for (int w = 0; w < kChunk / 64; w++) {
