@@ -244,6 +244,7 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5))
         S.keys[i] = dense && (uint64_t)i < words ? (uint32_t)i : kEmpty;
         S.vals[i] = 0u;
     }
+    __syncthreads();  // every slot is set before any lane inserts into the table
     uint64_t first = begin;
     do {
         const int count = (int)(end - first < (uint64_t)kChunk ? end - first : (uint64_t)kChunk);

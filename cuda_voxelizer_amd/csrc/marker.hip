@@ -122,7 +122,6 @@ __device__ __forceinline__ void column_run(uint32_t* __restrict__ table, int64_t
 template <bool MULTI, bool REG>
 __global__ void __launch_bounds__(kMBlock)
     solid_marker_kernel(ItemDev one, ItemMap M, MarkerWs ws, int spills, RegionArgs RA) {
-    __shared__ float s_tri[kMBlock * 9];
     __shared__ unsigned queue[kMQueue];
     __shared__ unsigned qn;
     __shared__ BlockJob J;
@@ -154,18 +153,11 @@ __global__ void __launch_bounds__(kMBlock)
             H.vals[i] = 0u;
         }
     }
-    const float* src = tris + first * 9;
-    stage_floats<kMBlock>(s_tri, src, count * 9);
-    __syncthreads();
+    __syncthreads();  // qn (and the LDS hash) set
     unsigned long long offrow = 0;
     if ((int)threadIdx.x < count) {
         float v[3][3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            v[k][0] = s_tri[threadIdx.x * 9 + 3 * k + 0] - info.bbox_min[0];
-            v[k][1] = s_tri[threadIdx.x * 9 + 3 * k + 1] - info.bbox_min[1];
-            v[k][2] = s_tri[threadIdx.x * 9 + 3 * k + 2] - info.bbox_min[2];
-        }
+        load_tri(tris, first + threadIdx.x, info, v);
         SolidTri S;
         solid_setup(v, info, S);
         if (S.valid) {
@@ -205,12 +197,7 @@ __global__ void __launch_bounds__(kMBlock)
     for (unsigned q = 0; q < nq; q++) {
         const int lt = (int)queue[q];
         float v[3][3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            v[k][0] = s_tri[lt * 9 + 3 * k + 0] - info.bbox_min[0];
-            v[k][1] = s_tri[lt * 9 + 3 * k + 1] - info.bbox_min[1];
-            v[k][2] = s_tri[lt * 9 + 3 * k + 2] - info.bbox_min[2];
-        }
+        load_tri(tris, first + (uint64_t)lt, info, v);  // the same triangle in every lane
         SolidTri S;
         solid_setup(v, info, S);
         const int ylo = (int)S.ymin < 0 ? 0 : (int)S.ymin, zlo = (int)S.zmin < 0 ? 0 : (int)S.zmin;
