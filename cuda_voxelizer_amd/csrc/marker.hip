@@ -272,15 +272,14 @@ __device__ __forceinline__ void suffix_rows_range(uint32_t* __restrict__ table, 
     }
 }
 
-// The same for G % 128 == 0 and a 16-byte aligned table: a lane holds 4
-// consecutive words (one 16-byte load), a row is rw/4 <= 64 lanes, a wave
-// holds 256/rw rows.
-// Rows as 16-byte groups, one lane per group, whole rows per wave: G a
-// multiple of 128 and at most 64 * 128, the table 16-byte aligned.
+// Where the 16-byte row form below applies: G a multiple of 128 and at most
+// 64 * 128 (whole rows per wave), the table 16-byte aligned.
 __host__ __device__ inline bool rows4_ok(uint32_t G, const uint32_t* table) {
     return (G & 127u) == 0 && G <= 8192u && (((uintptr_t)table) & 15u) == 0;
 }
 
+// The same for rows4_ok tables: a lane holds 4 consecutive words (one 16-byte
+// load), a row is rw/4 <= 64 lanes, a wave holds 256/rw rows.
 // CLEAR: the markers are read from `src` (the library's scratch marker table)
 // and the rows written to `table` whole; src words that held markers are
 // returned to zero, so the scratch table needs no fill before the next call.
