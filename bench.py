@@ -491,6 +491,7 @@ def run_batch(args) -> dict:
     t_tune = time.perf_counter()
     tuner = Tuner()
     tuner.tune_jobs(jobs)  # per-class algorithm and knobs, cached on disk per device
+    knobs_kept = tuner.settle(jobs)  # the knobs checked on the whole batch against the built-in settings
     t_tune = time.perf_counter() - t_tune
     algo_mix, knob_mix = {}, {}
     for j in jobs:
@@ -541,7 +542,7 @@ def run_batch(args) -> dict:
         "triangles_per_s": ntri / t_batch,
         "plain_meshes_per_s": len(meshes) / t_plain, "speedup_vs_plain": t_plain / t_batch,
         "batch_equals_plain": same, "batch_info": batch.info(), "autotune_s": t_tune, "algo_mix": algo_mix,
-        "knob_mix": knob_mix, "fixed_knobs_meshes_per_s": len(meshes) / t_fixed,
+        "knob_mix": knob_mix, "tuned_knobs_kept": knobs_kept, "fixed_knobs_meshes_per_s": len(meshes) / t_fixed,
         "gain_over_fixed_knobs": t_fixed / t_batch, "tuned_equals_fixed_knobs": same_fixed,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": alg_bytes,

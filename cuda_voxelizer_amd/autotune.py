@@ -27,7 +27,10 @@ waves per SIMD (5 or 6, two instantiations) and its queue threshold (voxel
 boxes above 64, 256 or 1024 voxels are shared by the workgroup), the marker
 kernel's queue threshold (16, 64 or 256 columns), and the tile algorithm's
 tile size (half, built-in, twice).  Every setting gives the same table; the
-chosen VOX_TUNE_* bits are cached with the algorithm.
+chosen VOX_TUNE_* bits are cached with the algorithm.  ``Tuner.settle`` then
+checks the knobs on the whole batch (tuned against built-in, same algorithms)
+and keeps the faster: knobs picked per class can split a mixed batch into more
+class kernels than it pays for.
 
 Cache file: $VOX_AUTOTUNE_CACHE, else ~/.cache/cuda_voxelizer_amd/autotune.json.
 """
@@ -222,6 +225,46 @@ class Tuner:
             entry["knob_seconds"] = knob_times
         self.table[key] = entry
         return algo, tune
+
+    def _time_jobs(self, jobs, tunes) -> float:
+        """Seconds of one graph launch of `jobs` with the VOX_TUNE_* bits `tunes`
+        (best of `reps`), into scratch tables so the jobs' own are untouched."""
+        import torch
+
+        from . import Batch, BatchJob
+        copies = [BatchJob(j.info, j.tris, torch.empty_like(j.table), morton=j.morton, solid=j.solid, algo=j.algo,
+                           tune=t) for j, t in zip(jobs, tunes)]
+        batch = Batch(copies, use_graph=True)
+        try:
+            batch.launch()  # warm (code load, workspaces)
+            torch.cuda.synchronize()
+            best = float("inf")
+            for _ in range(self.reps):
+                t0 = time.perf_counter()
+                batch.launch()
+                torch.cuda.synchronize()
+                best = min(best, time.perf_counter() - t0)
+        finally:
+            batch.close()
+        return best
+
+    def settle(self, jobs, batch_timer=None) -> bool:
+        """Whole-batch check of the tuned knobs.  Each class's knobs were timed
+        on copies of one job; in the mixed batch they can split the chunk jobs
+        into more classes, one kernel each, whose tails leave the GPU idle.  So
+        the batch is timed once with the tuned knobs and once with every knob at
+        its built-in setting (same algorithms), and the faster is kept.  Returns
+        True when the tuned knobs stay.  `batch_timer(jobs, tunes)` -> seconds
+        replaces the GPU timing."""
+        tuned = [int(getattr(j, "tune", 0)) for j in jobs]
+        if not any(tuned):
+            return True
+        timer = batch_timer or self._time_jobs
+        keep = timer(jobs, tuned) <= timer(jobs, [0] * len(jobs))
+        if not keep:
+            for j in jobs:
+                j.tune = 0
+        return keep
 
     def tune_jobs(self, jobs, knobs: bool = True) -> list:
         """Set every job's algorithm (and, with `knobs`, its VOX_TUNE_* bits)

@@ -169,10 +169,13 @@ static vox_status plan_batch(const vox_batch_item* items, int count, vox_batch* 
         } else if (a == CHUNK) {
             d.span = chunk_span(G, n);
             d.parts = chunk_parts(G, n);
-            // jobs that loop over chunks run in the 4-wave form (any layout): they stay in the 5-wave class
+            // jobs that loop over chunks run in the 4-wave form (any layout): they stay in the 5-wave class;
+            // a layout-specialised kernel runs at six waves per SIMD under either bound, so only
+            // layout-generic jobs are split by the waves knob (each class is one more kernel)
             const bool loop = d.span > 1u && d.parts <= 1u;
             const int sp = loop ? (int)kSpecAny : chunk_spec(it.layout, G);
-            JobClass& c = (tune_waves(it.flags) == 6 && !loop) ? b->chunkw[sp] : b->chunk[sp];
+            const bool wide = tune_waves(it.flags) == 6 && !loop && sp == kSpecAny;
+            JobClass& c = wide ? b->chunkw[sp] : b->chunk[sp];
             if (loop) c.loop = true;
             c.items.push_back(d);
             add_prefix(c.base_a, c.total_a, chunk_units(n, d.span, d.parts));

@@ -117,3 +117,34 @@ def test_knobs_tuned_one_axis_at_a_time_and_cached(tmp_path):
     job3.tune = 7
     Tuner(path, device_name="fake", timer=timer).tune_jobs([job3], knobs=False)
     assert job3.algo == "chunk" and job3.tune == 0  # the fixed defaults, for comparison
+
+
+def test_settle_keeps_the_faster_knobs_for_the_whole_batch(tmp_path):
+    """Knobs tuned per class are checked on the whole batch: they stay when the
+    batch is faster with them, and go back to the built-in settings otherwise."""
+    from cuda_voxelizer_amd.autotune import Tuner
+    t = Tuner(str(tmp_path / "t.json"), device_name="fake", timer=lambda job, algo, tune=0: 1.0)
+    seen = []
+
+    def make_jobs():
+        jobs = [_job(64, False, 1000), _job(128, True, 5000), _job(32, False, 200)]
+        for j, tune in zip(jobs, (0x2000000, 0x1000000, 0)):
+            j.tune = tune
+        return jobs
+
+    def timer_fixed_faster(jobs, tunes):
+        seen.append(list(tunes))
+        return 2.0 if any(tunes) else 1.5
+
+    jobs = make_jobs()
+    assert t.settle(jobs, batch_timer=timer_fixed_faster) is False
+    assert [j.tune for j in jobs] == [0, 0, 0]
+    assert seen == [[0x2000000, 0x1000000, 0], [0, 0, 0]]  # tuned first, then built-in
+
+    jobs = make_jobs()
+    assert t.settle(jobs, batch_timer=lambda jobs, tunes: 1.0 if any(tunes) else 1.5) is True
+    assert [j.tune for j in jobs] == [0x2000000, 0x1000000, 0]
+
+    untuned = [_job(64, False, 1000)]
+    untuned[0].tune = 0
+    assert t.settle(untuned, batch_timer=lambda jobs, tunes: 1 / 0) is True  # nothing to check, nothing timed
