@@ -36,6 +36,8 @@ struct vox_batch {
     int device = 0;
     bool use_graph = true;
     hipStream_t cap_stream = nullptr;
+    hipStream_t side[3] = {};  // capture only: tile chain, marker chain, direct classes
+    hipEvent_t ev[5] = {};     // capture only: fork, tables zeroed, and the three side chains' ends
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void* dev_mem = nullptr;
@@ -56,10 +58,18 @@ static void add_prefix(std::vector<uint32_t>& base, uint32_t& total, uint64_t un
     base.push_back(total);
 }
 
-static hipError_t launch_all(vox_batch* b, hipStream_t s) {
+// The classes of a batch, by the launch that runs them.  Every job is in one
+// algorithm class; the fill class zeroes the tables of the chunk, marker and
+// direct jobs that do not accumulate (tile jobs write every word themselves).
+static hipError_t launch_fill_class(vox_batch* b, hipStream_t s) {
+    const ItemDev dummy{};
+    if (!b->fill.total_a) return hipSuccess;
+    return launch_fill_jobs(dummy, b->fill.m_a, b->fill.total_a, s);
+}
+
+static hipError_t launch_chunk_classes(vox_batch* b, hipStream_t s) {
     const ItemDev dummy{};
     hipError_t e = hipSuccess;
-    if (b->fill.total_a && (e = launch_fill_jobs(dummy, b->fill.m_a, b->fill.total_a, s)) != hipSuccess) return e;
     for (int sp = 0; sp < 3; sp++) {
         const JobClass& c5 = b->chunk[sp];
         const JobClass& c6 = b->chunkw[sp];
@@ -68,14 +78,29 @@ static hipError_t launch_all(vox_batch* b, hipStream_t s) {
         if (c6.total_a && (e = launch_chunk_jobs(dummy, c6.m_a, c6.total_a, c6.loop, s, 6, sp)) != hipSuccess)
             return e;
     }
-    if (!b->marker.items.empty() &&
-        (e = launch_marker_jobs(b->marker.m_a, b->marker.m_b, b->marker.m_c, b->marker.total_a, b->marker.total_b,
-                                b->mws, s)) != hipSuccess)
-        return e;
+    return hipSuccess;
+}
+
+static hipError_t launch_marker_class(vox_batch* b, hipStream_t s) {
+    if (b->marker.items.empty()) return hipSuccess;
+    return launch_marker_jobs(b->marker.m_a, b->marker.m_b, b->marker.m_c, b->marker.total_a, b->marker.total_b,
+                              b->mws, s);
+}
+
+static hipError_t launch_direct_classes(vox_batch* b, hipStream_t s) {
+    const ItemDev dummy{};
+    hipError_t e = hipSuccess;
     if (b->dsurf.total_a && (e = launch_direct_jobs(dummy, b->dsurf.m_a, b->dsurf.total_a, false, s)) != hipSuccess)
         return e;
     if (b->dsolid.total_a && (e = launch_direct_jobs(dummy, b->dsolid.m_a, b->dsolid.total_a, true, s)) != hipSuccess)
         return e;
+    return hipSuccess;
+}
+
+// The two tile classes share the binning workspace: one after the other.
+static hipError_t launch_tile_classes(vox_batch* b, hipStream_t s) {
+    const ItemDev dummy{};
+    hipError_t e = hipSuccess;
     if (!b->tsurf.items.empty() &&
         (e = launch_tiled_jobs(dummy, b->tsurf.m_a, b->tsurf.m_b, b->tsurf.m_c, b->tsurf.total_a, b->tsurf.total_b,
                                false, b->tsurf.max_tile_words, b->ws, b->L, s)) != hipSuccess)
@@ -87,11 +112,57 @@ static hipError_t launch_all(vox_batch* b, hipStream_t s) {
     return hipSuccess;
 }
 
+// Every class in stream order (eager launches and runners).
+static hipError_t launch_all(vox_batch* b, hipStream_t s) {
+    hipError_t e = hipSuccess;
+    if ((e = launch_fill_class(b, s)) != hipSuccess) return e;
+    if ((e = launch_chunk_classes(b, s)) != hipSuccess) return e;
+    if ((e = launch_marker_class(b, s)) != hipSuccess) return e;
+    if ((e = launch_direct_classes(b, s)) != hipSuccess) return e;
+    return launch_tile_classes(b, s);
+}
+
+// For graph capture on `m`: classes that touch different jobs run as
+// concurrent branches, so one class's tail (its last workgroups) and the
+// small kernels of the tile and marker chains overlap the others instead of
+// leaving the GPU idle between kernels.  The tile chain needs no fill and
+// starts at once; the marker chain and the direct classes follow the fill;
+// the chunk classes run on `m`, which joins every branch at the end.  Tables
+// of different jobs do not overlap, so the branches share no data (the
+// out-of-row counters they add to are atomic).
+static hipError_t launch_forked(vox_batch* b, hipStream_t m) {
+    hipError_t e = hipSuccess;
+#define VOX_TRY(x) \
+    if ((e = (x)) != hipSuccess) return e
+    VOX_TRY(hipEventRecord(b->ev[0], m));
+    VOX_TRY(hipStreamWaitEvent(b->side[0], b->ev[0], 0));
+    VOX_TRY(launch_tile_classes(b, b->side[0]));
+    VOX_TRY(hipEventRecord(b->ev[2], b->side[0]));
+    VOX_TRY(launch_fill_class(b, m));
+    VOX_TRY(hipEventRecord(b->ev[1], m));
+    VOX_TRY(hipStreamWaitEvent(b->side[1], b->ev[1], 0));
+    VOX_TRY(launch_marker_class(b, b->side[1]));
+    VOX_TRY(hipEventRecord(b->ev[3], b->side[1]));
+    VOX_TRY(hipStreamWaitEvent(b->side[2], b->ev[1], 0));
+    VOX_TRY(launch_direct_classes(b, b->side[2]));
+    VOX_TRY(hipEventRecord(b->ev[4], b->side[2]));
+    VOX_TRY(launch_chunk_classes(b, m));
+    VOX_TRY(hipStreamWaitEvent(m, b->ev[2], 0));
+    VOX_TRY(hipStreamWaitEvent(m, b->ev[3], 0));
+    VOX_TRY(hipStreamWaitEvent(m, b->ev[4], 0));
+#undef VOX_TRY
+    return hipSuccess;
+}
+
 static void free_batch(vox_batch* b) {
     if (!b) return;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    for (hipStream_t& st : b->side)
+        if (st) (void)hipStreamDestroy(st);
+    for (hipEvent_t& ev : b->ev)
+        if (ev) (void)hipEventDestroy(ev);
     if (b->dev_mem) (void)hipFree(b->dev_mem);
     if (b->ws) (void)hipFree(b->ws);
     if (b->mws) (void)hipFree(b->mws);
@@ -298,9 +369,13 @@ vox_status vox_batch_create(const vox_batch_item* items, int count, unsigned fla
     }
     if (b->use_graph) {
         e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
+        for (hipStream_t& st : b->side)
+            if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        for (hipEvent_t& ev : b->ev)
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
         if (e == hipSuccess) e = hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            hipError_t le = launch_all(b, b->cap_stream);
+            hipError_t le = launch_forked(b, b->cap_stream);
             e = hipStreamEndCapture(b->cap_stream, &b->graph);
             if (le != hipSuccess) e = le;
         }
