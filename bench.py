@@ -254,6 +254,9 @@ def main() -> None:
         "data": "synthetic",
     }
     out.update({k: v for k, v in main_w.items() if k not in ("value", "unit", "ms_per_step")})
+    copy_gbs = measured_copy_gbs()
+    out["roofline"]["measured_copy_GBs"] = copy_gbs
+    out["roofline"]["frac_of_measured_copy"] = out["roofline"]["achieved"] / copy_gbs
     extras = [w for w in args.extra.split(",") if w and w != args.config] if args.extra else []
     if extras:
         out["extra_workloads"] = {w: run_workload(w, args, min(args.steps, 20), min(args.warmup, 3),
@@ -279,6 +282,26 @@ def run_copycal(args) -> dict:
     dt = (time.perf_counter() - t0) / k
     return {"metric": "copy GB/s (PMC calibration)", "value": 2 * n / dt / 1e9, "unit": "GB/s", "bytes_read": n,
             "bytes_written": n, "steps": k, "ms_per_step": dt * 1e3}
+
+
+def measured_copy_gbs(nbytes: int = 256 << 20, reps: int = 20) -> float:
+    """HBM bandwidth a plain device-to-device copy reaches on this GPU (bytes
+    read + written per second, HIP events on torch's stream): the practical
+    ceiling beside the 8 TB/s specification."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e.record()
+    torch.cuda.synchronize()
+    gbs = 2 * nbytes * reps / (s.elapsed_time(e) * 1e-3) / 1e9
+    del src, dst
+    return gbs
 
 
 def _event_ms(fn, steps: int, warmup: int, stream) -> float:
