@@ -72,6 +72,45 @@ def test_batch_matches_oracle(oracle, gpu, use_graph):
     b.close()
 
 
+def test_graph_replays_match_the_eager_batch(oracle, gpu):
+    """The graph runs its classes as concurrent branches (tile chain, marker
+    chain and direct classes beside the chunk classes).  Replayed four times,
+    with eager work and synchronisations of the program's own between the
+    replays, every replay's tables and out-of-row counts equal those of the
+    same batch launched eagerly (classes in stream order), which equal the
+    oracle's."""
+    import torch
+    import cuda_voxelizer_amd as V
+    jobs, host = _jobs(gpu, count=30, seed=0xB5)
+    eager = V.Batch(jobs, use_graph=False)
+    info = eager.info()
+    assert info["chunk_jobs"] > 0 and info["tile_surface_jobs"] + info["tile_solid_jobs"] > 0
+    assert info["direct_jobs"] > 0
+    eager.launch()
+    _check(oracle, jobs, host)
+    ref = [j.table.clone() for j in jobs]
+    ref_off = [int(j.offrow.item()) if j.offrow is not None else None for j in jobs]
+    graph = V.Batch(jobs, use_graph=True)
+    other = torch.zeros(1 << 20, dtype=torch.int32, device=gpu)
+    for rep in range(4):
+        for j in jobs:
+            j.table.fill_(-1)
+            if j.offrow is not None:
+                j.offrow.zero_()
+        other.add_(rep + 1)  # unrelated eager work between the replays
+        torch.cuda.synchronize()
+        print(f"graph replay {rep}", flush=True)
+        graph.launch()
+        torch.cuda.synchronize()
+        for j, t, o in zip(jobs, ref, ref_off):
+            assert torch.equal(j.table, t), (rep, j.info, j.morton, j.solid, j.algo)
+            if o is not None:
+                assert int(j.offrow.item()) == o, (rep, j.algo)
+    assert int(other[0].item()) == 10
+    graph.close()
+    eager.close()
+
+
 def test_batch_sees_new_triangle_contents(oracle, gpu):
     import torch
     import cuda_voxelizer_amd as V
