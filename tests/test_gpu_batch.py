@@ -111,6 +111,41 @@ def test_graph_replays_match_the_eager_batch(oracle, gpu):
     eager.close()
 
 
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_empty_and_one_branch_batches(oracle, gpu, use_graph):
+    """Batches whose graph has empty branches: no jobs at all, only tile jobs
+    (the branch that needs no fill), only direct jobs, only chunk jobs (the
+    main stream's own branch); each launches twice and matches the oracle."""
+    import torch
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import meshgen
+    empty = V.Batch([], use_graph=use_graph)
+    empty.launch()
+    empty.launch()
+    torch.cuda.synchronize()
+    assert empty.info()["jobs"] == 0
+    empty.close()
+    v, f = meshgen.icosphere_sh(level=3, seed=7)
+    tris = meshgen.to_aos(v, f)
+    for algo, solid in (("tile", False), ("tile", True), ("direct", False), ("direct", True), ("chunk", False)):
+        print(f"one-branch batch algo={algo} solid={solid} graph={use_graph}", flush=True)
+        jobs, host = [], []
+        for g in (32, 64):
+            info = V.voxinfo_for_mesh(tris.reshape(-1, 3), g, tris.shape[0])
+            d = torch.from_numpy(np.ascontiguousarray(tris).reshape(-1)).to(gpu)
+            t = torch.full((V.table_words(g),), -1, dtype=torch.int32, device=gpu)
+            jobs.append(V.BatchJob(info, d, t, solid=solid, algo=algo))
+            host.append(tris)
+        b = V.Batch(jobs, use_graph=use_graph)
+        b.launch()
+        _check(oracle, jobs, host)
+        for j in jobs:
+            j.table.fill_(-1)
+        b.launch()
+        _check(oracle, jobs, host)
+        b.close()
+
+
 def test_batch_sees_new_triangle_contents(oracle, gpu):
     import torch
     import cuda_voxelizer_amd as V
