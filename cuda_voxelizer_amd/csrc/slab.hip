@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(kSBlock) slab_count_kernel(vox_info info, cons
     for (int it = 0; it < kBinIters; it++) {
         const uint64_t c0 = first + (uint64_t)it * kSBlock;
         if (c0 >= n) break;  // uniform over the workgroup
-        __syncthreads();     // the previous round has read s_tri
+        __syncthreads();     // the previous iteration has read s_tri
 #pragma unroll
         for (int u = 0; u < 9; u++) s_tri[threadIdx.x + u * kSBlock] = pre[u];
         if (it + 1 < kBinIters) fetch(c0 + kSBlock);  // in flight during this round

@@ -28,8 +28,8 @@
  *   VOXO_MODE_REF  : omp parallel for over triangles, one global critical
  *                    section per bit write (the reference's structure; this is
  *                    the timed CPU baseline).
- *   VOXO_MODE_FAST : same loops, lock-free __atomic OR/XOR per bit (used as
- *                    the checker at large sizes).
+ *   VOXO_MODE_FAST : same loops, lock-free __atomic OR/XOR per bit (the
+ *                    expected tables of the parity tests at large sizes).
  *
  * Out-of-row runs (documented, counted).  The reference's point-in-triangle
  * test accepts column centres within an absolute 1e-6 (world units^2) of an
