@@ -226,6 +226,54 @@ def test_solid_row_widths(oracle, gpu, grid):
     assert np.unpackbits(got.view(np.uint8)).any()
 
 
+def _sparse_gpu(t):
+    """(indices, words) of the nonzero words of an int32 device table, in pieces
+    of 2^30 words (torch's nonzero takes at most INT_MAX elements)."""
+    import torch
+    idx, val = [], []
+    piece = 1 << 30
+    for o in range(0, t.numel(), piece):
+        part = t[o:o + piece]
+        i = torch.nonzero(part).flatten()
+        idx.append(i.cpu().numpy() + o)
+        val.append(part[i].cpu().numpy().view(np.uint32))
+    return np.concatenate(idx), np.concatenate(val)
+
+
+@pytest.mark.parametrize("solid,morton", [(False, False), (False, True), (True, False)])
+def test_grid_4096_small_sphere(oracle, gpu, solid, morton):
+    """4096^3 (8 GiB tables, 2^31 words): a small sphere in a large box, so the
+    linear chunk kernel's 32-bit word index, the 64-bit Morton codes above
+    1024 and the streaming solid pass over 2^24 rows all run at that size.
+    Compared as (word index, word) pairs of the nonzero words."""
+    import torch
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import meshgen
+    grid = 4096
+    v, f = meshgen.icosphere_sh(level=4, seed=4096)
+    v = (v * np.float32(0.12) + np.float32(3.1)).astype(np.float32)  # near the far corner of [0, 4]^3
+    tris = meshgen.to_aos(v, f)
+    box = V.AABox(np.array([0, 0, 0], np.float32), np.array([4, 4, 4], np.float32))
+    info = V.VoxInfo(box, grid, f.shape[0])
+    print(f"G=4096 solid={solid} morton={morton}", flush=True)
+    d = _dev_tris(tris, gpu)
+    t = torch.full((V.table_words(grid),), -1, dtype=torch.int32, device=gpu)
+    if solid:
+        V.voxelize_solid_table(info, d, morton=morton, table=t)
+        exp, _, _ = oracle.solid(_oracle_info(oracle, info), tris, morton=morton)
+    else:
+        V.voxelize_surface(info, d, morton=morton, table=t)
+        exp, _ = oracle.surface(_oracle_info(oracle, info), tris, morton=morton)
+    torch.cuda.synchronize()
+    gi, gw = _sparse_gpu(t)
+    del t
+    ei = np.flatnonzero(exp)
+    ew = exp[ei]
+    del exp
+    assert ei.size > 1000
+    assert gi.size == ei.size and np.array_equal(gi, ei) and np.array_equal(gw, ew)
+
+
 def test_solid_morton_2048_small_sphere(oracle, gpu):
     """Morton solid at 2048^3: the marker table is assembled into bricks from
     blocks of 32 x 2 rows of 16 quads each (the row suffix taken on the loaded
