@@ -278,12 +278,10 @@ __device__ __forceinline__ void morton_chunk_words(uint64_t base_code, uint32_t 
         // spread10(4 g) for g < 8: x bits 2, 3, 4 land in code bits 6, 9, 12
         const uint64_t code = base_code | (uint64_t)((((uint32_t)g & 1u) << 6) | (((uint32_t)g & 2u) << 8) |
                                                      (((uint32_t)g & 4u) << 10));
-        uint32_t bits = 0;
-        const uint32_t j0 = yz_low;  // j for x offset 0 in the group
-        if (nib & 1u) bits |= 1u << (31u - j0);
-        if (nib & 2u) bits |= 1u << (31u - (j0 | 1u));
-        if (nib & 4u) bits |= 1u << (31u - (j0 | 8u));
-        if (nib & 8u) bits |= 1u << (31u - (j0 | 9u));
+        // x offsets 0..3 of the group are code bits 0 and 3 (j0 | 0, 1, 8, 9; yz_low
+        // never has bits 0 or 3), i.e. word bits 31, 30, 23, 22 shifted down by j0
+        const uint32_t bits = (((nib & 1u) << 31) | ((nib & 2u) << 29) | ((nib & 4u) << 21) | ((nib & 8u) << 19)) >>
+                              yz_low;
         fn(code >> 5, bits);
     }
 }
