@@ -260,7 +260,13 @@ class Tuner:
         if not any(tuned):
             return True
         timer = batch_timer or self._time_jobs
-        keep = timer(jobs, tuned) <= timer(jobs, [0] * len(jobs))
+        fixed = [0] * len(jobs)
+        # alternate the two settings and keep each one's best, so that a drift of
+        # the device between timings (clocks, caches) favours neither
+        t_tuned, t_fixed = timer(jobs, tuned), timer(jobs, fixed)
+        t_fixed = min(t_fixed, timer(jobs, fixed))
+        t_tuned = min(t_tuned, timer(jobs, tuned))
+        keep = t_tuned <= t_fixed
         if not keep:
             for j in jobs:
                 j.tune = 0

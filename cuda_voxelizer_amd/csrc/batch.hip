@@ -36,8 +36,8 @@ struct vox_batch {
     int device = 0;
     bool use_graph = true;
     hipStream_t cap_stream = nullptr;
-    hipStream_t side[3] = {};  // capture only: tile chain, marker chain, direct classes
-    hipEvent_t ev[5] = {};     // capture only: fork, tables zeroed, and the three side chains' ends
+    hipStream_t side[4] = {};  // capture only: tile chain, marker chain, direct classes, generic chunk classes
+    hipEvent_t ev[6] = {};     // capture only: fork, tables zeroed, and the four side branches' ends
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void* dev_mem = nullptr;
@@ -67,10 +67,13 @@ static hipError_t launch_fill_class(vox_batch* b, hipStream_t s) {
     return launch_fill_jobs(dummy, b->fill.m_a, b->fill.total_a, s);
 }
 
-static hipError_t launch_chunk_classes(vox_batch* b, hipStream_t s) {
+// which: -1 every chunk class; kSpecAny only the layout-generic classes (the
+// chunk loop of small dense tables among them); 1 only the layout-specialised.
+static hipError_t launch_chunk_classes(vox_batch* b, hipStream_t s, int which = -1) {
     const ItemDev dummy{};
     hipError_t e = hipSuccess;
     for (int sp = 0; sp < 3; sp++) {
+        if ((which == kSpecAny && sp != kSpecAny) || (which == 1 && sp == kSpecAny)) continue;
         const JobClass& c5 = b->chunk[sp];
         const JobClass& c6 = b->chunkw[sp];
         if (c5.total_a && (e = launch_chunk_jobs(dummy, c5.m_a, c5.total_a, c5.loop, s, 5, sp)) != hipSuccess)
@@ -124,32 +127,37 @@ static hipError_t launch_all(vox_batch* b, hipStream_t s) {
 
 // For graph capture on `m`: classes that touch different jobs run as
 // concurrent branches, so one class's tail (its last workgroups) and the
-// small kernels of the tile and marker chains overlap the others instead of
-// leaving the GPU idle between kernels.  The tile chain needs no fill and
-// starts at once; the marker chain and the direct classes follow the fill;
-// the chunk classes run on `m`, which joins every branch at the end.  Tables
-// of different jobs do not overlap, so the branches share no data (the
-// out-of-row counters they add to are atomic).
+// small latency-bound kernels (the tile chain's scans, the chunk loop of
+// small dense tables, the marker chain's spill pass) overlap the others
+// instead of leaving the GPU idle between kernels.  Nodes are created in the
+// order the critical path wants them dispatched: the fill, then the
+// layout-specialised chunk classes (the bulk of most batches) on `m`, then
+// the branches -- the tile chain (no fill; it forks from before the fill),
+// the marker chain, the direct classes and the layout-generic chunk classes
+// (after the fill).  `m` joins every branch at the end.  Tables of different
+// jobs do not overlap, so the branches share no data (the out-of-row
+// counters they add to are atomic).
 static hipError_t launch_forked(vox_batch* b, hipStream_t m) {
     hipError_t e = hipSuccess;
 #define VOX_TRY(x) \
     if ((e = (x)) != hipSuccess) return e
     VOX_TRY(hipEventRecord(b->ev[0], m));
+    VOX_TRY(launch_fill_class(b, m));
+    VOX_TRY(hipEventRecord(b->ev[1], m));
+    VOX_TRY(launch_chunk_classes(b, m, 1));
     VOX_TRY(hipStreamWaitEvent(b->side[0], b->ev[0], 0));
     VOX_TRY(launch_tile_classes(b, b->side[0]));
     VOX_TRY(hipEventRecord(b->ev[2], b->side[0]));
-    VOX_TRY(launch_fill_class(b, m));
-    VOX_TRY(hipEventRecord(b->ev[1], m));
     VOX_TRY(hipStreamWaitEvent(b->side[1], b->ev[1], 0));
     VOX_TRY(launch_marker_class(b, b->side[1]));
     VOX_TRY(hipEventRecord(b->ev[3], b->side[1]));
     VOX_TRY(hipStreamWaitEvent(b->side[2], b->ev[1], 0));
     VOX_TRY(launch_direct_classes(b, b->side[2]));
     VOX_TRY(hipEventRecord(b->ev[4], b->side[2]));
-    VOX_TRY(launch_chunk_classes(b, m));
-    VOX_TRY(hipStreamWaitEvent(m, b->ev[2], 0));
-    VOX_TRY(hipStreamWaitEvent(m, b->ev[3], 0));
-    VOX_TRY(hipStreamWaitEvent(m, b->ev[4], 0));
+    VOX_TRY(hipStreamWaitEvent(b->side[3], b->ev[1], 0));
+    VOX_TRY(launch_chunk_classes(b, b->side[3], kSpecAny));
+    VOX_TRY(hipEventRecord(b->ev[5], b->side[3]));
+    for (int k = 2; k < 6; k++) VOX_TRY(hipStreamWaitEvent(m, b->ev[k], 0));
 #undef VOX_TRY
     return hipSuccess;
 }

@@ -139,7 +139,8 @@ def test_settle_keeps_the_faster_knobs_for_the_whole_batch(tmp_path):
     jobs = make_jobs()
     assert t.settle(jobs, batch_timer=timer_fixed_faster) is False
     assert [j.tune for j in jobs] == [0, 0, 0]
-    assert seen == [[0x2000000, 0x1000000, 0], [0, 0, 0]]  # tuned first, then built-in
+    tuned = [0x2000000, 0x1000000, 0]
+    assert seen == [tuned, [0, 0, 0], [0, 0, 0], tuned]  # alternating, each setting twice
 
     jobs = make_jobs()
     assert t.settle(jobs, batch_timer=lambda jobs, tunes: 1.0 if any(tunes) else 1.5) is True
