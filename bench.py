@@ -491,11 +491,12 @@ def run_rows(args) -> dict:
 def _batch_jobs(meshes, dev):
     import torch
     import cuda_voxelizer_amd as V
+    # the tables first, one after the other, then the meshes
+    tables = [torch.empty(V.table_words(m.grid), dtype=torch.int32, device=dev) for m in meshes]
     jobs = []
-    for m in meshes:
+    for m, t in zip(meshes, tables):
         info = V.voxinfo_for_mesh(m.tris.reshape(-1, 3), m.grid, m.tris.shape[0])
         d = torch.from_numpy(m.tris.reshape(-1)).to(dev)
-        t = torch.empty(V.table_words(m.grid), dtype=torch.int32, device=dev)
         jobs.append(V.BatchJob(info, d, t, morton=False, solid=m.solid))
     return jobs
 
@@ -522,9 +523,8 @@ def run_batch(args) -> dict:
         k = f"{j.algo}:{describe_tune(j.algo, j.tune)}"
         knob_mix[k] = knob_mix.get(k, 0) + 1
     batch = V.Batch(jobs, use_graph=True)
-    # the same algorithms with every knob at its built-in setting, for the knobs' gain
-    fixed_jobs = [V.BatchJob(j.info, j.tris, torch.empty_like(j.table), morton=j.morton, solid=j.solid, algo=j.algo)
-                  for j in jobs]
+    # the same algorithms and tables with every knob at its built-in setting, for the knobs' gain
+    fixed_jobs = [V.BatchJob(j.info, j.tris, j.table, morton=j.morton, solid=j.solid, algo=j.algo) for j in jobs]
     fixed = V.Batch(fixed_jobs, use_graph=True)
     steps = max(1, min(args.steps, 10))
 
@@ -551,8 +551,12 @@ def run_batch(args) -> dict:
     t_fixed = timed(lambda: fixed.launch(stream), steps)
     t_fixed = min(t_fixed, timed(lambda: fixed.launch(stream), steps))
     t_batch = min(t_batch, timed(lambda: batch.launch(stream), steps))
+    batch.launch(stream)
+    torch.cuda.synchronize()
     batch_tables = [j.table.clone() for j in jobs]
-    same_fixed = all(bool(torch.equal(a.table, b)) for a, b in zip(fixed_jobs, batch_tables))
+    fixed.launch(stream)
+    torch.cuda.synchronize()
+    same_fixed = all(bool(torch.equal(j.table, b)) for j, b in zip(jobs, batch_tables))
     fixed.close()
     t_plain = timed(plain, steps)
     same = all(bool((a == j.table).all()) for a, j in zip(batch_tables, jobs))
