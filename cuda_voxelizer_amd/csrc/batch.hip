@@ -67,13 +67,16 @@ static hipError_t launch_fill_class(vox_batch* b, hipStream_t s) {
     return launch_fill_jobs(dummy, b->fill.m_a, b->fill.total_a, s);
 }
 
-// which: -1 every chunk class; kSpecAny only the layout-generic classes (the
-// chunk loop of small dense tables among them); 1 only the layout-specialised.
-static hipError_t launch_chunk_classes(vox_batch* b, hipStream_t s, int which = -1) {
+// Which chunk classes a launch covers: all, the layout-generic ones (the
+// chunk loop of small dense tables among them) or the layout-specialised ones.
+enum ChunkClassSet { kAllChunkClasses, kGenericChunkClasses, kSpecialisedChunkClasses };
+
+static hipError_t launch_chunk_classes(vox_batch* b, hipStream_t s, ChunkClassSet which = kAllChunkClasses) {
     const ItemDev dummy{};
     hipError_t e = hipSuccess;
     for (int sp = 0; sp < 3; sp++) {
-        if ((which == kSpecAny && sp != kSpecAny) || (which == 1 && sp == kSpecAny)) continue;
+        if ((which == kGenericChunkClasses && sp != kSpecAny) || (which == kSpecialisedChunkClasses && sp == kSpecAny))
+            continue;
         const JobClass& c5 = b->chunk[sp];
         const JobClass& c6 = b->chunkw[sp];
         if (c5.total_a && (e = launch_chunk_jobs(dummy, c5.m_a, c5.total_a, c5.loop, s, 5, sp)) != hipSuccess)
@@ -144,7 +147,7 @@ static hipError_t launch_forked(vox_batch* b, hipStream_t m) {
     VOX_TRY(hipEventRecord(b->ev[0], m));
     VOX_TRY(launch_fill_class(b, m));
     VOX_TRY(hipEventRecord(b->ev[1], m));
-    VOX_TRY(launch_chunk_classes(b, m, 1));
+    VOX_TRY(launch_chunk_classes(b, m, kSpecialisedChunkClasses));
     VOX_TRY(hipStreamWaitEvent(b->side[0], b->ev[0], 0));
     VOX_TRY(launch_tile_classes(b, b->side[0]));
     VOX_TRY(hipEventRecord(b->ev[2], b->side[0]));
@@ -155,7 +158,7 @@ static hipError_t launch_forked(vox_batch* b, hipStream_t m) {
     VOX_TRY(launch_direct_classes(b, b->side[2]));
     VOX_TRY(hipEventRecord(b->ev[4], b->side[2]));
     VOX_TRY(hipStreamWaitEvent(b->side[3], b->ev[1], 0));
-    VOX_TRY(launch_chunk_classes(b, b->side[3], kSpecAny));
+    VOX_TRY(launch_chunk_classes(b, b->side[3], kGenericChunkClasses));
     VOX_TRY(hipEventRecord(b->ev[5], b->side[3]));
     for (int k = 2; k < 6; k++) VOX_TRY(hipStreamWaitEvent(m, b->ev[k], 0));
 #undef VOX_TRY
