@@ -252,8 +252,9 @@ class Tuner:
         """Whole-batch check of the tuned knobs.  Each class's knobs were timed
         on copies of one job; in the mixed batch they can split the chunk jobs
         into more classes, one kernel each, whose tails leave the GPU idle.  So
-        the batch is timed once with the tuned knobs and once with every knob at
-        its built-in setting (same algorithms), and the faster is kept.  Returns
+        the batch is timed with the tuned knobs and with every knob at its
+        built-in setting (same algorithms), twice each in the order tuned,
+        fixed, fixed, tuned, and the setting with the better best is kept.  Returns
         True when the tuned knobs stay.  `batch_timer(jobs, tunes)` -> seconds
         replaces the GPU timing."""
         tuned = [int(getattr(j, "tune", 0)) for j in jobs]
