@@ -12,6 +12,7 @@
 // int32[3*nf] -- released with vox_free_mesh.
 #include <cctype>
 #include <cerrno>
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -133,7 +134,8 @@ std::string load_off(const std::string& data, Mesh& m) {
     if (tok.size() < 4) return "truncated OFF header";
     const long nv = strtol(tok[t].c_str(), nullptr, 10), nf = strtol(tok[t + 1].c_str(), nullptr, 10);
     t += 3;
-    if (nv < 0 || nf < 0 || tok.size() < t + 3 * (size_t)nv) return "truncated OFF vertex list";
+    if (nv < 0 || nf < 0 || (size_t)nv > tok.size() / 3 || tok.size() < t + 3 * (size_t)nv)
+        return "truncated OFF vertex list";
     for (long i = 0; i < 3 * nv; i++) m.v.push_back(strtof(tok[t++].c_str(), nullptr));
     std::vector<int32_t> poly;
     for (long i = 0; i < nf; i++) {
@@ -221,6 +223,18 @@ struct PElem {
     long n = 0;
     std::vector<PProp> props;
 };
+
+// binary64 -> binary32 rounded to nearest, without a conversion whose source
+// lies outside the float range (undefined in C++): values from FLT_MAX + half
+// an ulp up round to infinity, those just above FLT_MAX to FLT_MAX.
+float to_float(double d) {
+    const double top = (double)FLT_MAX, lim = top + 0x1p103;
+    if (d >= lim) return INFINITY;
+    if (d <= -lim) return -INFINITY;
+    if (d > top) return FLT_MAX;
+    if (d < -top) return -FLT_MAX;
+    return (float)d;  // NaN passes through
+}
 
 // binary reader: returns the value as double (exact for every PLY type)
 struct Bin {
@@ -322,6 +336,7 @@ std::string load_ply(const std::string& data, Mesh& m) {
             if (el.props[i].name == "z") iz = (int)i;
         }
         if (is_v && (ix < 0 || iy < 0 || iz < 0)) return "PLY vertex element without x/y/z";
+        if (el.props.empty()) continue;  // rows of nothing: no data to skip
         for (long r = 0; r < el.n; r++) {
             float xyz[3] = {0, 0, 0};
             for (size_t i = 0; i < el.props.size(); i++) {
@@ -344,6 +359,8 @@ std::string load_ply(const std::string& data, Mesh& m) {
                             if (!next_tok(tok)) return "truncated PLY body";
                             idx = strtod(tok.c_str(), nullptr);
                         }
+                        if (!bin.ok) return "truncated PLY body";
+                        if (!(idx >= 0.0 && idx < 2147483648.0)) return "PLY face index out of range";
                         poly.push_back((int32_t)idx);
                     }
                     if (!bin.ok) return "truncated PLY body";
@@ -351,7 +368,7 @@ std::string load_ply(const std::string& data, Mesh& m) {
                 } else {
                     float v;
                     if (fmt) {
-                        v = (float)bin.get(p.type);  // binary32 stays exact; binary64 rounds once
+                        v = to_float(bin.get(p.type));  // binary32 stays exact; binary64 rounds once
                         if (!bin.ok) return "truncated PLY body";
                     } else {
                         if (!next_tok(tok)) return "truncated PLY body";

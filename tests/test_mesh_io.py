@@ -144,3 +144,70 @@ def test_cli_options(tmp_path, capsys):
     with pytest.raises(SystemExit) as e:
         cli.parse(["-h"])
     assert e.value.code == 0
+
+
+_PLY_HEAD = ("ply\nformat ascii 1.0\nelement vertex 3\nproperty float x\nproperty float y\nproperty float z\n"
+             "{extra}element face 1\nproperty list uchar int vertex_indices\nend_header\n0 0 0\n1 0 0\n0 1 0\n{face}\n")
+
+
+@pytest.mark.parametrize("name,data", [
+    # 3 * nv wraps 64 bits back to a small number
+    ("wrap.off", "OFF\n6148914691236517206 1 0\n0 0 0\n1 0 0\n"),
+    ("huge_nv.off", "OFF\n9223372036854775807 0 0\n0 0 0\n"),
+    # face indices past int32 or not numbers at all
+    ("big.ply", _PLY_HEAD.format(extra="", face="3 0 1 3623878656")),
+    ("neg.ply", _PLY_HEAD.format(extra="", face="3 0 1 -1")),
+    ("nan.ply", _PLY_HEAD.format(extra="", face="3 0 1 nan")),
+    ("inf.ply", _PLY_HEAD.format(extra="", face="3 0 1 1e300")),
+])
+def test_out_of_range_counts_and_indices_are_errors(capi, tmp_path, name, data):
+    """Counts and indices beyond every limit give an error: no wrapped bound,
+    and no float-to-int conversion of an out-of-range index."""
+    from cuda_voxelizer_amd import io
+    from cuda_voxelizer_amd._capi import VoxError
+    p = tmp_path / name
+    p.write_text(data)
+    with pytest.raises(VoxError):
+        io.load_mesh(str(p))
+
+
+def test_ply_binary_index_past_int32_is_an_error(capi, tmp_path):
+    from cuda_voxelizer_amd import io
+    from cuda_voxelizer_amd._capi import VoxError
+    p = tmp_path / "b.ply"
+    with open(p, "wb") as fh:
+        fh.write(b"ply\nformat binary_little_endian 1.0\nelement vertex 3\nproperty float x\nproperty float y\n"
+                 b"property float z\nelement face 1\nproperty list uchar uint vertex_indices\nend_header\n")
+        fh.write(np.zeros(9, "<f4").tobytes() + b"\x03" + np.array([0, 1, 2**31], "<u4").tobytes())
+    with pytest.raises(VoxError):
+        io.load_mesh(str(p))
+
+
+def test_ply_element_without_properties_is_skipped(capi, tmp_path):
+    """An element with no properties has no data, whatever its count."""
+    from cuda_voxelizer_amd import io
+    p = tmp_path / "e.ply"
+    p.write_text(_PLY_HEAD.format(extra="element junk 1000000000000000000\n", face="3 0 1 2"))
+    v, f = io.load_mesh(str(p))
+    assert v.shape == (3, 3) and f.tolist() == [[0, 1, 2]]
+
+
+def test_ply_binary64_coordinates_round_to_nearest_float(capi, tmp_path):
+    """binary64 coordinates round once to binary32, overflow included: past
+    FLT_MAX by less than half an ulp (2^103) gives FLT_MAX, from half an ulp
+    on infinity (the tie rounds to even), as numpy's IEEE conversion does."""
+    from cuda_voxelizer_amd import io
+    fmax = float(np.finfo(np.float32).max)
+    xyz = np.array([[fmax + 2.0**102, -(fmax + 2.0**102), 1e300],
+                    [fmax + 2.0**103, -1e300, 1.0 / 3.0],
+                    [0.1, -2.5e-46, 5e-324]], "<f8")
+    p = tmp_path / "d.ply"
+    with open(p, "wb") as fh:
+        fh.write(b"ply\nformat binary_little_endian 1.0\nelement vertex 3\nproperty double x\nproperty double y\n"
+                 b"property double z\nelement face 1\nproperty list uchar int vertex_indices\nend_header\n")
+        fh.write(xyz.tobytes() + b"\x03" + np.array([0, 1, 2], "<i4").tobytes())
+    v, _ = io.load_mesh(str(p))
+    with np.errstate(over="ignore"):
+        exp = xyz.astype(np.float32)
+    assert exp[0, 0] == np.float32(fmax) and np.isinf(exp[1, 0])
+    np.testing.assert_array_equal(v, exp)
