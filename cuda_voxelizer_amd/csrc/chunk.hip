@@ -1,8 +1,9 @@
 // chunk.hip -- single-pass surface voxelization with LDS word merging.
 //
-// One workgroup takes a chunk of 256 consecutive triangles (meshes are stored
-// with spatially coherent triangle order, so a chunk covers a compact
-// region).  Each lane loads and rasterizes its own triangle (ZX edges once
+// One workgroup takes a chunk of 256 consecutive triangles; the fewer chunks
+// share a table word, the fewer atomics, so a mesh whose consecutive
+// triangles lie close together voxelizes fastest (its chunks are compact
+// patches; input order is kept as given).  Each lane loads and rasterizes its own triangle (ZX edges once
 // per z slice, then per row the YZ edges and, at the x that passed ZX, the
 // plane and XY edges) and ORs the passing bits into an LDS hash table keyed
 // by table word, so bits that several triangles of the chunk set in one word
