@@ -3,11 +3,12 @@
 // One workgroup takes a chunk of 256 consecutive triangles; the fewer chunks
 // share a table word, the fewer atomics, so a mesh whose consecutive
 // triangles lie close together voxelizes fastest (its chunks are compact
-// patches; input order is kept as given).  Each lane loads and rasterizes its own triangle (ZX edges once
-// per z slice, then per row the YZ edges and, at the x that passed ZX, the
-// plane and XY edges) and ORs the passing bits into an LDS hash table keyed
-// by table word, so bits that several triangles of the chunk set in one word
-// are merged before memory is touched.  Triangles with a large voxel box are
+// patches; input order is kept as given).  Each lane loads and rasterizes
+// its own triangle (ZX edges once per z slice, then per row the YZ edges
+// and, at the x that passed ZX, the plane and XY edges) and ORs the passing
+// bits into an LDS hash table keyed by table word, so bits that several
+// triangles of the chunk set in one word are merged before memory is
+// touched.  Triangles with a large voxel box are
 // queued; afterwards their owners publish test records to LDS (up to kRecs
 // per round) and the workgroup deals the rounds' (row, 32-voxel x chunk)
 // items evenly over its lanes, testing only the x range that the plane band
