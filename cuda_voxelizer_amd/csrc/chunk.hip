@@ -8,11 +8,11 @@
 // and, at the x that passed ZX, the plane and XY edges) and ORs the passing
 // bits into an LDS hash table keyed by table word, so bits that several
 // triangles of the chunk set in one word are merged before memory is
-// touched.  Triangles with a large voxel box are
-// queued; afterwards their owners publish test records to LDS (up to kRecs
-// per round) and the workgroup deals the rounds' (row, 32-voxel x chunk)
-// items evenly over its lanes, testing only the x range that the plane band
-// leaves open.  The table is then updated with one non-returning global
+// touched.  Triangles with a large voxel box (or one far larger than the
+// rest of their wave's) are queued; afterwards their owners publish test
+// records to LDS (up to kRecs per round) and the workgroup deals the rounds'
+// (row, 32-voxel x chunk) items evenly over its lanes, testing only the x
+// range that the plane band leaves open.  The table is then updated with one non-returning global
 // atomicOr per distinct word; words that do not fit in the hash go straight
 // to global atomics.  The table is zero-filled beforehand (unless
 // accumulating).
@@ -34,6 +34,16 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 // more than the atomic it would save.
 constexpr int kProbes = 2;
 constexpr int kBigVox = 256;    // voxel boxes above this (or wider than 32 in x) are queued
+// ...and, with the built-in setting, boxes of more than kRowsFloor rows and
+// more than kRowsRatio times the mean of their wave's boxes: one lane walking
+// a box far larger than its neighbours' holds up the whole workgroup.  The
+// Stanford bunny at 64^3 (5,110 triangles, 20 workgroups): the median
+// workgroup's largest lane box has 40 rows against a mean of 7.4; queuing
+// those 1.7 % of triangles halves it.  The icospheres, tori and displaced
+// spheres of the benchmarks have no box above the floor, so their waves skip
+// the rule after one ballot.
+constexpr int kRowsFloor = 16;
+constexpr int kRowsRatio = 3;
 constexpr int kRecs = 48;       // queued triangles whose records are in LDS at once
 
 struct ChunkShared {
@@ -255,14 +265,26 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5))
         // served by the cache after the first; staging the chunk through LDS
         // cost a barrier and 9 KB of LDS (512^3 Morton torus 51.2 vs 48.0 us)
         bool big = false;
+        int rows = 0;  // rows of a lane-path box
         SurfTri T;
         float v[3][3];
         if ((int)threadIdx.x < count) {
             load_tri(it.tris, first + threadIdx.x, info, v);
             surface_box(v, info, T.lo, T.hi);
             const int nx = T.hi[0] - T.lo[0] + 1;
-            const int64_t vol = (int64_t)nx * (T.hi[1] - T.lo[1] + 1) * (T.hi[2] - T.lo[2] + 1);
+            const int ny = T.hi[1] - T.lo[1] + 1, nz = T.hi[2] - T.lo[2] + 1;
+            const int64_t vol = (int64_t)nx * ny * nz;
             big = vol > (it.big ? (int64_t)it.big : (int64_t)kBigVox) || nx > 32;
+            rows = big ? 0 : ny * nz;  // below kBigVox here, so no overflow
+        }
+        if (!it.big && __ballot(rows > kRowsFloor)) {
+            // sum over the wave (dead and queued lanes add 0); the mean is
+            // taken over the lanes that hold a triangle
+            int sum = rows;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+            const int live = (int)__popcll(__ballot((int)threadIdx.x < count));
+            if (rows > kRowsFloor && rows * live > kRowsRatio * sum) big = true;
         }
         const unsigned long long bal = __ballot(big);
         const int wave = (int)(threadIdx.x >> 6);

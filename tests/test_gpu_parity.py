@@ -473,3 +473,34 @@ def test_small_table_dense_merge_chunk(oracle, gpu, grid):
     for morton in ((False, True) if grid & (grid - 1) == 0 else (False,)):
         got, exp = _run(oracle, gpu, tris, grid, morton, False, "chunk")
         assert _mismatch(got, exp) == 0, morton
+
+
+@pytest.mark.parametrize("grid", [64, 96, 128])
+def test_chunk_wave_outliers_queued(oracle, gpu, grid):
+    """With the built-in queue setting the chunk kernel also queues a box of
+    more than 16 rows and more than 3x its wave's mean (boxes of at most 256
+    voxels otherwise stay on their lane).  Here every wave of 64 triangles holds
+    tiny triangles and one to four thin ones, at random lanes, whose boxes are
+    1 or 2 voxels wide in x and 13 to 17 rows in y and z: about 70 of them per
+    grid are queued by the wave rule alone (their boxes have at most 256
+    voxels), the rest by their volume.  The table must equal the oracle's, and
+    so must the one computed with the absolute 256-voxel rule alone
+    (VOX_TUNE_BIG(8))."""
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import _capi, meshgen
+    rng = np.random.default_rng(grid)
+    tiny = meshgen.triangle_soup(64 * 40, seed=grid, mu=-6.0, sigma=0.5).reshape(-1, 64, 3, 3)
+    u = 1.0 / grid
+    for w in range(tiny.shape[0]):
+        for lane in rng.choice(64, size=1 + w % 4, replace=False):
+            x = rng.uniform(0.1, 0.9)
+            y0, z0 = rng.uniform(0.05, 0.7, size=2)
+            h = u * rng.integers(12, 16)  # 13-17 rows in y and z
+            tiny[w, lane] = [[x, y0, z0], [x + 0.3 * u, y0 + h, z0], [x, y0 + 0.5 * h, z0 + h]]
+    tris = tiny.reshape(-1, 3, 3).astype(np.float32)
+    for morton in ((False, True) if grid & (grid - 1) == 0 else (False,)):
+        got, exp = _run(oracle, gpu, tris, grid, morton, False, "chunk")
+        assert _mismatch(got, exp) == 0, morton
+        info = _info(tris.reshape(-1, 3), grid, tris.shape[0])
+        t = V.voxelize_surface(info, _dev_tris(tris, gpu), morton=morton, algo="chunk", tune=_capi.tune_big(8))
+        assert _mismatch(_host_table(t), exp) == 0, morton
