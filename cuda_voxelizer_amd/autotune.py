@@ -24,8 +24,9 @@ launch noise) is ruled out, and the fastest per copy among the rest wins.
 Once the algorithm is chosen, its knobs are tuned the same way, one axis at
 a time starting from the built-in setting (``KNOBS``): the chunk kernel's
 waves per SIMD (5 or 6, two instantiations) and its queue threshold (voxel
-boxes above 64, 256 or 1024 voxels are shared by the workgroup), the marker
-kernel's queue threshold (16, 64 or 256 columns), and the tile algorithm's
+boxes above 64 or 1024 voxels are shared by the workgroup, or the built-in
+rule: above 256, or far larger than their wave's mean), the marker kernel's
+queue threshold (16, 32, 64 or 256 columns), and the tile algorithm's
 tile size (half, built-in, twice).  Every setting gives the same table; the
 chosen VOX_TUNE_* bits are cached with the algorithm.  ``Tuner.settle`` then
 checks the knobs on the whole batch (tuned against built-in, same algorithms)
@@ -50,8 +51,8 @@ def _knobs():
     # algorithm -> axes, each a list of (label, VOX_TUNE_* bits); 0 is the built-in setting
     return {
         "chunk": [[("waves=5", tune_waves(1)), ("waves=6", tune_waves(2))],
-                  [("big=64", tune_big(6)), ("big=256", 0), ("big=1024", tune_big(10))]],
-        "marker": [[("big=16", tune_big(4)), ("big=64", 0), ("big=256", tune_big(8))]],
+                  [("big=64", tune_big(6)), ("big=built-in", 0), ("big=1024", tune_big(10))]],
+        "marker": [[("big=16", tune_big(4)), ("big=32", 0), ("big=64", tune_big(6)), ("big=256", tune_big(8))]],
         "tile": [[("tile=half", tune_tile(1)), ("tile=1", 0), ("tile=twice", tune_tile(3))]],
         "direct": [],
     }

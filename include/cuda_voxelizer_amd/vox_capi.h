@@ -106,7 +106,8 @@ typedef enum vox_status {
 /* Tuning bits for the default algorithms (autotuner knobs; a zero field is the
    built-in choice; every setting gives the same table):
    VOX_TUNE_BIG(lg)    chunk: voxel boxes above 2^lg voxels are shared by the workgroup
-                       (built-in 256); marker: triangles above 2^lg columns (built-in 64)
+                       (built-in 256, plus boxes far larger than their wave's mean); marker: triangles
+                       above 2^lg columns (built-in 32)
    VOX_TUNE_WAVES(w)   chunk: 1 = 5 waves per SIMD, 2 = 6 (built-in: 6 linear, 5 Morton)
    VOX_TUNE_TILE(t)    tile: 1 = half the built-in tile, 3 = twice (within the LDS limit) */
 #define VOX_TUNE_BIG(lg) ((((unsigned)(lg) + 1u) & 15u) << 24)
