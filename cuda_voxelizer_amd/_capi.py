@@ -146,6 +146,7 @@ SIGNATURES = {
     "vox_mesh_bbox": (ctypes.c_int, [_P, ctypes.c_uint64, _P, _P]),
     "vox_pack_triangles": (ctypes.c_int, [_P, ctypes.c_uint64, _P, ctypes.c_uint64, _P, _P, _P]),
     "vox_upload_async": (ctypes.c_int, [_P, ctypes.c_size_t, _P, _P]),
+    "vox_sort_triangles": (ctypes.c_int, [ctypes.POINTER(vox_info), _P, _P, _P, _P]),
     "vox_convert_layout": (ctypes.c_int, [_P, _P, ctypes.c_uint32, ctypes.c_int, _P]),
     "vox_count_voxels": (ctypes.c_int, [_P, ctypes.c_uint64, _P, _P]),
     "vox_check_voxel": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,

@@ -178,6 +178,21 @@ vox_status vox_mesh_bbox(const float* d_vertices, uint64_t n_vertices, float* d_
    are counted into *d_bad (added to; may be NULL). */
 vox_status vox_pack_triangles(const float* d_vertices, uint64_t n_vertices, const int32_t* d_faces,
                               uint64_t n_faces, float* d_tris, uint64_t* d_bad, void* stream);
+/* Spatial order for a mesh that is voxelized many times (no counterpart in the
+   reference; for callers that keep meshes resident, such as serving loops).
+   d_out[9*n] receives the triangles of d_tris (n = info->n_triangles) bucketed
+   by the 16^3-voxel brick, at info's grid, of each triangle's lowest voxel
+   corner, the bricks in Morton order; d_perm[n] (may be NULL) the index in
+   d_tris of each output triangle.  Every voxelization of d_out, at any grid,
+   layout and algorithm, gives the same table as of d_tris; the chunk
+   algorithm merges more of its words in LDS and issues fewer atomics (1024^3
+   torus of 1 M triangles 0.076 vs 0.106 ms per call, an icosphere whose faces
+   interleave 0.113 vs 0.322 ms).  Out of place: d_out and d_perm must not
+   overlap d_tris or each other.  The order within a brick is unspecified.
+   The library keeps per-stream counts and keys (4 bytes per triangle and up
+   to 2 MiB), allocated at the first call or when a larger one comes. */
+vox_status vox_sort_triangles(const vox_info* info, const float* d_tris, float* d_out, uint32_t* d_perm,
+                              void* stream);
 /* Host -> device copy through double-buffered pinned staging (returns once the
    last chunk is queued; the copies are ordered on `stream`). */
 vox_status vox_upload_async(const void* host, size_t bytes, void* d_dst, void* stream);

@@ -123,3 +123,115 @@ def test_side_stream_calls_are_ordered(oracle, gpu):
         side.synchronize()
         exp_m, _ = oracle.surface(oracle.info_from_bytes(info.as_bytes()), (v, f), morton=True)
         assert (mor.cpu().numpy().view(np.uint32) == exp_m).all()
+
+
+# ---- spatial order (vox_sort_triangles) ---------------------------------------------
+def _brick_keys(tris, info):
+    """Host restatement of the sort key: the voxel box's lowest corner at info's
+    grid (the kernels' binary32 arithmetic: v - bbox_min, min over the
+    vertices, / unit, truncated, clamped), in bricks of 16 voxels (2^(lg G - 6)
+    above 1024), Morton-interleaved."""
+    G = info.grid
+    bmin = np.asarray(info.bbox.min, np.float32)
+    unit = info.unit.astype(np.float32)
+    lo = ((tris.reshape(-1, 3, 3) - bmin).min(axis=1) / unit)
+    lo = np.clip(np.trunc(lo), 0, G - 1).astype(np.int64)
+    lg = int(np.ceil(np.log2(G))) if G > 1 else 0
+    shift = max(4, lg - 6)
+    b = lo >> shift
+    key = np.zeros(len(b), np.int64)
+    for bit in range(7):
+        for ax in range(3):
+            key |= ((b[:, ax] >> bit) & 1) << (3 * bit + ax)
+    return key
+
+
+def _check_sorted(oracle, gpu, tris, grid, layouts=(False,), solid=(False, True), algos=("auto", "chunk")):
+    import torch
+    import cuda_voxelizer_amd as V
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 3, 3)
+    n = tris.shape[0]
+    info = V.voxinfo_for_mesh(tris.reshape(-1, 3), grid, n)
+    d = torch.from_numpy(tris.reshape(-1)).to(gpu)
+    print(f"sort case: n={n} grid={grid}", flush=True)
+    out, perm = V.sort_triangles(info, d, with_perm=True)
+    p = perm.cpu().numpy().astype(np.int64)
+    o = out.cpu().numpy().reshape(-1, 3, 3)
+    assert (np.sort(p) == np.arange(n)).all()  # a permutation
+    assert (o.view(np.uint32) == tris[p].view(np.uint32)).all()  # the same triangles, bit for bit
+    k = _brick_keys(o, info)
+    assert (np.diff(k) >= 0).all()  # bricks in Morton order
+    oi = oracle.info_from_bytes(info.as_bytes())
+    for morton in layouts:
+        for s in solid:
+            exp = oracle.solid(oi, tris, morton=morton)[0] if s else oracle.surface(oi, tris, morton=morton)[0]
+            for algo in algos:
+                if s and algo == "chunk":
+                    continue
+                if s:
+                    got = V.voxelize_solid_table(info, out, morton=morton, algo=algo)
+                else:
+                    got = V.voxelize_surface(info, out, morton=morton, algo=algo)
+                assert (got.cpu().numpy().view(np.uint32) == exp).all(), (morton, s, algo)
+
+
+def test_sort_triangles_torus_512(oracle, gpu):
+    from cuda_voxelizer_amd import meshgen
+    _check_sorted(oracle, gpu, meshgen.torus_slivers(grid=512), 512, layouts=(False, True), solid=(False,))
+
+
+def test_sort_triangles_interleaved_icosphere(oracle, gpu):
+    """Subdivision order scatters consecutive faces over the whole sphere."""
+    from cuda_voxelizer_amd import meshgen
+    v, f = meshgen.icosphere_sh(level=7)
+    _check_sorted(oracle, gpu, meshgen.to_aos(v, f), 256, layouts=(False, True))
+
+
+@pytest.mark.parametrize("grid,n", [(1, 1000), (16, 5000), (33, 70001), (100, 1), (2048, 3000)])
+def test_sort_triangles_grids_and_counts(oracle, gpu, grid, n):
+    """One brick (8 buckets, the minimum), partial last bricks, a ragged last
+    wave and workgroup, a single triangle, and a grid above 1024 (32-voxel
+    bricks)."""
+    from cuda_voxelizer_amd import meshgen
+    tris = meshgen.triangle_soup(n, seed=grid + n, mu=-4.0, sigma=1.0)
+    _check_sorted(oracle, gpu, tris, grid, algos=("chunk",) if grid < 2048 else ("auto",),
+                  solid=(False,) if grid == 2048 else (False, True))
+
+
+def test_sort_triangles_back_to_back_and_side_stream(oracle, gpu):
+    """The per-stream counts must be zero again after every call: sizes and
+    bucket counts change from call to call, and one call runs on a side stream."""
+    import torch
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import meshgen
+    v, f = meshgen.icosphere_sh(level=6)
+    ico = meshgen.to_aos(v, f)
+    soup = meshgen.triangle_soup(120_000, seed=4, mu=-5.0, sigma=1.0)
+    for tris, grid in ((ico, 512), (soup, 16), (soup, 1000), (ico, 64)):
+        _check_sorted(oracle, gpu, tris, grid, solid=(False,), algos=("chunk",))
+    s = torch.cuda.Stream()
+    info = V.voxinfo_for_mesh(ico.reshape(-1, 3), 256, ico.shape[0])
+    d = torch.from_numpy(ico.reshape(-1)).to(gpu)
+    out = V.sort_triangles(info, d, stream=s)
+    s.synchronize()
+    got = V.voxelize_surface(info, out, algo="chunk").cpu().numpy().view(np.uint32)
+    exp, _ = oracle.surface(oracle.info_from_bytes(info.as_bytes()), ico)
+    assert (got == exp).all()
+
+
+def test_sort_triangles_rejects_overlap_and_sorts_nothing(gpu):
+    import ctypes
+    import torch
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import _capi, meshgen
+    tris = meshgen.triangle_soup(1000, seed=2)
+    info = V.voxinfo_for_mesh(tris.reshape(-1, 3), 64, tris.shape[0])
+    d = torch.from_numpy(tris.reshape(-1)).to(gpu)
+    lib = _capi.load()
+    with pytest.raises(_capi.VoxError, match="overlaps"):
+        _capi.check(lib.vox_sort_triangles(ctypes.byref(info._c), d.data_ptr(), d.data_ptr() + 36 * 500, None, None))
+    out = torch.empty_like(d)
+    with pytest.raises(_capi.VoxError, match="overlaps"):
+        _capi.check(lib.vox_sort_triangles(ctypes.byref(info._c), d.data_ptr(), out.data_ptr(), out.data_ptr(), None))
+    empty = V.VoxInfo(V.AABox([0, 0, 0], [1, 1, 1]), (32, 32, 32), 0)
+    assert V.sort_triangles(empty, d).numel() == 0
