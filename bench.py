@@ -330,6 +330,24 @@ def _row(name, ms, alg_bytes, cpu_ms, cpu_sample, same, **extra):
     return out
 
 
+def _brick_keys(tris, info):
+    """Sort key of vox_sort_triangles restated in numpy: the voxel box's lowest
+    corner at info's grid (binary32: v - bbox_min, min over the vertices, / unit,
+    truncated, clamped), in bricks of 16 voxels (2^(lg G - 6) above 1024),
+    Morton-interleaved."""
+    G = info.grid
+    lo = ((np.asarray(tris, np.float32).reshape(-1, 3, 3) - np.asarray(info.bbox.min, np.float32)).min(axis=1)
+          / info.unit.astype(np.float32))
+    lo = np.clip(np.trunc(lo), 0, G - 1).astype(np.int64)
+    shift = max(4, (int(np.ceil(np.log2(G))) if G > 1 else 0) - 6)
+    b = lo >> shift
+    key = np.zeros(len(b), np.int64)
+    for bit in range(7):
+        for ax in range(3):
+            key |= ((b[:, ax] >> bit) & 1) << (3 * bit + ax)
+    return key
+
+
 def run_rows(args) -> dict:
     """The rows around the voxelization path, each timed on the GPU against the
     HBM roofline with its CPU restatement timed beside it and its output checked
@@ -342,6 +360,10 @@ def run_rows(args) -> dict:
                  body length, so its time includes one host round trip;
       convert -- linear -> Morton re-layout of the 1024^3 linear torus table
                  (voxelize.cuh:20-34), bytes 2 G^3/8, and back;
+      sort    -- spatial order of the 1024^3 torus's triangles (no counterpart
+                 in the reference), bytes 76 per triangle (read, write, index),
+                 with the surface call on the torus and on the interleaved
+                 icosphere before and after;
       cli     -- the command line end to end (read OBJ, voxelize 256^3, write
                  binvox) on the reference's test model, wall time."""
     import ctypes
@@ -453,6 +475,44 @@ def run_rows(args) -> dict:
                      f"numpy permutation of a {g}^3 table, scaled x{(G // g) ** 3} to {G}^3", same,
                      morton_to_linear_ms=back_ms,
                      morton_to_linear_frac=2 * G ** 3 / 8 / (back_ms * 1e-3) / 1e9 / HBM_PEAK_GBS))
+
+    # sort: spatial order of the 1024^3 torus (vox_sort_triangles), and what it buys
+    # the chunk algorithm on that torus and on an icosphere whose faces interleave
+    n = mesh.shape[0]
+    s_out = torch.empty_like(d_tris)
+    s_perm = torch.empty(n, dtype=torch.int32, device=dev)
+
+    def sort():
+        _capi.check(lib.vox_sort_triangles(ctypes.byref(info._c), ctypes.c_void_p(d_tris.data_ptr()),
+                                           ctypes.c_void_p(s_out.data_ptr()), ctypes.c_void_p(s_perm.data_ptr()), sp))
+    ms = _event_ms(sort, steps, args.warmup, stream)
+    perm_h = s_perm.cpu().numpy().astype(np.int64)
+    out_h = s_out.cpu().numpy().reshape(-1, 3, 3)
+    t0 = time.perf_counter()
+    ref_keys = _brick_keys(mesh, info)
+    ref_order = np.argsort(ref_keys, kind="stable")
+    ref_sorted = mesh[ref_order]
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    same = (np.array_equal(np.sort(perm_h), np.arange(n)) and np.array_equal(out_h.view(np.uint32),
+                                                                               mesh[perm_h].view(np.uint32))
+            and np.array_equal(np.sort(_brick_keys(out_h, info)), _brick_keys(ref_sorted, info))
+            and bool((np.diff(_brick_keys(out_h, info)) >= 0).all()))
+    order_gain = {}
+    v8, f8 = meshgen.icosphere_sh()
+    for name, m_tris in (("torus1024", mesh), ("icosphere_sh_1024", meshgen.to_aos(v8, f8))):
+        m_info = V.voxinfo_for_mesh(m_tris.reshape(-1, 3), G, m_tris.shape[0])
+        m_in = torch.from_numpy(np.ascontiguousarray(m_tris).reshape(-1)).to(dev)
+        m_sorted = V.sort_triangles(m_info, m_in)
+        t_in = torch.empty(V.table_words(G), dtype=torch.int32, device=dev)
+        t_so = torch.empty_like(t_in)
+        ms_in = _event_ms(lambda: V.voxelize_surface(m_info, m_in, table=t_in), steps, args.warmup, stream)
+        ms_so = _event_ms(lambda: V.voxelize_surface(m_info, m_sorted, table=t_so), steps, args.warmup, stream)
+        ms_in = min(ms_in, _event_ms(lambda: V.voxelize_surface(m_info, m_in, table=t_in), steps, args.warmup, stream))
+        order_gain[name] = {"input_order_ms": ms_in, "sorted_ms": ms_so, "tables_identical": bool(torch.equal(t_in, t_so))}
+        same = same and order_gain[name]["tables_identical"]
+    rows.append(_row("sort_triangles", ms, 76 * n, cpu_ms,
+                     f"numpy brick keys + stable argsort + gather, {n} triangles, 1 thread", same,
+                     voxelize_surface_1024=order_gain))
 
     # cli: end to end on the reference's test model
     from cuda_voxelizer_amd import cli

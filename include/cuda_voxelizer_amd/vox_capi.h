@@ -186,8 +186,8 @@ vox_status vox_pack_triangles(const float* d_vertices, uint64_t n_vertices, cons
    d_tris of each output triangle.  Every voxelization of d_out, at any grid,
    layout and algorithm, gives the same table as of d_tris; the chunk
    algorithm merges more of its words in LDS and issues fewer atomics (1024^3
-   torus of 1 M triangles 0.076 vs 0.106 ms per call, an icosphere whose faces
-   interleave 0.113 vs 0.322 ms).  Out of place: d_out and d_perm must not
+   torus of 1 M triangles 0.083 vs 0.107 ms per call, an icosphere whose faces
+   interleave 0.126 vs 0.323 ms; the sort of the torus takes 0.036 ms).  Out of place: d_out and d_perm must not
    overlap d_tris or each other.  The order within a brick is unspecified.
    The library keeps per-stream counts and keys (4 bytes per triangle and up
    to 2 MiB), allocated at the first call or when a larger one comes. */
