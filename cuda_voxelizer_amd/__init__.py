@@ -382,8 +382,9 @@ def sort_triangles(info: VoxInfo, tris, stream=None, with_perm: bool = False):
     for a mesh that is voxelized many times (vox_sort_triangles): a new tensor
     bucketed by the 16^3-voxel brick, at info's grid, of each triangle's lowest
     voxel corner, bricks in Morton order.  Every voxelization of it gives the
-    same table as of `tris`, faster with the chunk algorithm when consecutive
-    input triangles lie far apart.  With `with_perm`, also an int32 tensor:
+    same table as of `tris`, faster with the chunk algorithm when small
+    consecutive input triangles lie far apart, slower for soups of large
+    overlapping triangles.  With `with_perm`, also an int32 tensor:
     the index in `tris` of each output triangle."""
     torch = _torch()
     _check_tensor(tris, "torch.float32", "tris")
