@@ -235,3 +235,21 @@ def test_sort_triangles_rejects_overlap_and_sorts_nothing(gpu):
         _capi.check(lib.vox_sort_triangles(ctypes.byref(info._c), d.data_ptr(), out.data_ptr(), out.data_ptr(), None))
     empty = V.VoxInfo(V.AABox([0, 0, 0], [1, 1, 1]), (32, 32, 32), 0)
     assert V.sort_triangles(empty, d).numel() == 0
+
+
+def test_prepare_mesh_spatial_order(oracle, gpu):
+    """prepare_mesh(..., spatial_order=True): the same info, the triangles of the
+    packed mesh in brick order, and the oracle's tables in both modes."""
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import meshgen
+    v, f = meshgen.icosphere_sh(level=5, seed=9)
+    info, tris = V.prepare_mesh(v, f, 128, spatial_order=True)
+    assert info.as_bytes() == V.voxinfo_for_mesh(v, 128, f.shape[0]).as_bytes()
+    got_tris = tris.cpu().numpy().reshape(-1, 3, 3)
+    ref = meshgen.to_aos(v, f)
+    key = lambda a: np.sort(a.reshape(len(a), -1).view(np.uint32).view([("", np.uint32)] * 9), axis=0)
+    assert (key(got_tris) == key(ref)).all()  # the same multiset of triangles
+    assert (np.diff(_brick_keys(got_tris, info)) >= 0).all()
+    oi = oracle.info_from_bytes(info.as_bytes())
+    assert (V.voxelize_surface(info, tris).cpu().numpy().view(np.uint32) == oracle.surface(oi, (v, f))[0]).all()
+    assert (V.voxelize_solid_table(info, tris).cpu().numpy().view(np.uint32) == oracle.solid(oi, (v, f))[0]).all()
