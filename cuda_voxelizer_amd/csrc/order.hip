@@ -76,7 +76,7 @@ __device__ __forceinline__ void wave_runs(uint32_t key, bool live, bool& start, 
 }
 
 __global__ void __launch_bounds__(256) order_keys_kernel(const float* __restrict__ tris, vox_info info, uint64_t n,
-                                                         int shift, uint32_t* __restrict__ keys,
+                                                         int shift, uint32_t key_mask, uint32_t* __restrict__ keys,
                                                          uint32_t* __restrict__ counts) {
     const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     const bool live = t < n;
@@ -88,6 +88,9 @@ __global__ void __launch_bounds__(256) order_keys_kernel(const float* __restrict
         surface_box(v, info, lo, hi);  // clamped to [0, G-1]
         key = spread10((uint32_t)lo[0] >> shift) | (spread10((uint32_t)lo[1] >> shift) << 1) |
               (spread10((uint32_t)lo[2] >> shift) << 2);
+        // below the bucket count already (the box is clamped to the grid); the
+        // mask keeps the index in range whatever a NaN vertex made of the box
+        key &= key_mask;
         keys[t] = key;
     }
     bool start;
@@ -208,7 +211,7 @@ static hipError_t sort_triangles(const vox_info& info, const float* tris, float*
         b.dirty = true;  // until the scan kernel is queued
     }
     const unsigned blocks = (unsigned)((n + 255) / 256);
-    order_keys_kernel<<<blocks, 256, 0, stream>>>(tris, info, n, g.shift, keys, counts);
+    order_keys_kernel<<<blocks, 256, 0, stream>>>(tris, info, n, g.shift, g.buckets - 1u, keys, counts);
     order_totals_kernel<<<g.groups, 256, 0, stream>>>(counts, group_totals, g.buckets);
     order_scan_kernel<<<g.groups, 256, 0, stream>>>(counts, cursor, group_totals, g.buckets);
     order_scatter_kernel<<<blocks, 256, 0, stream>>>(tris, keys, n, cursor, out, perm);

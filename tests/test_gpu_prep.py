@@ -253,3 +253,22 @@ def test_prepare_mesh_spatial_order(oracle, gpu):
     oi = oracle.info_from_bytes(info.as_bytes())
     assert (V.voxelize_surface(info, tris).cpu().numpy().view(np.uint32) == oracle.surface(oi, (v, f))[0]).all()
     assert (V.voxelize_solid_table(info, tris).cpu().numpy().view(np.uint32) == oracle.solid(oi, (v, f))[0]).all()
+
+
+def test_sort_triangles_non_finite_vertices(gpu):
+    """Vertices that are NaN or infinite still give a permutation of the input,
+    bit for bit (their keys are clamped or masked into the bucket range)."""
+    import torch
+    import cuda_voxelizer_amd as V
+    from cuda_voxelizer_amd import meshgen
+    tris = meshgen.triangle_soup(5000, seed=8, mu=-4.0)
+    info = V.voxinfo_for_mesh(tris.reshape(-1, 3), 256, tris.shape[0])  # box of the finite mesh
+    bad = tris.copy()
+    bad[::7, 0, 0] = np.nan
+    bad[3::11, 1, 1] = np.inf
+    bad[5::13, 2, 2] = -np.inf
+    d = torch.from_numpy(bad.reshape(-1)).to(gpu)
+    out, perm = V.sort_triangles(info, d, with_perm=True)
+    p = perm.cpu().numpy().astype(np.int64)
+    assert (np.sort(p) == np.arange(len(bad))).all()
+    assert (out.cpu().numpy().reshape(-1, 3, 3).view(np.uint32) == bad[p].view(np.uint32)).all()
