@@ -222,9 +222,8 @@ static vox_status plan_batch(const vox_batch_item* items, int count, vox_batch* 
         else if (algo & VOX_FLAG_ALGO_TILE) a = have_plan ? TILE : DIRECT;
         else if (algo & VOX_FLAG_ALGO_CHUNK) a = chunk_supported(G) ? CHUNK : DIRECT;
         else if (algo & VOX_FLAG_ALGO_MARKER) a = have_marker ? MARKER : (have_plan ? TILE : DIRECT);
-        else if (solid && have_marker && !(have_plan && prefer_tile_surface(it.info.n_triangles, G))) a = MARKER;
+        else if (solid && have_marker && !(have_plan && coarse_mesh(it.info.n_triangles, G))) a = MARKER;
         else if (solid) a = have_plan ? TILE : DIRECT;
-        else if (have_plan && prefer_tile_surface(it.info.n_triangles, G)) a = TILE;
         else a = chunk_supported(G) ? CHUNK : (have_plan ? TILE : DIRECT);
         ItemDev d{};
         d.info = it.info;

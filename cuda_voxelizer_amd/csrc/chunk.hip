@@ -438,7 +438,7 @@ uint32_t chunk_span(uint32_t G, uint64_t n_triangles) {
 // chunks of large triangles: its queued items are split over up to 32
 // workgroups per chunk, aiming at 256 workgroups per job.
 uint32_t chunk_parts(uint32_t G, uint64_t n_triangles) {
-    if (!prefer_tile_surface(n_triangles, G)) return 1u;
+    if (!coarse_mesh(n_triangles, G)) return 1u;
     const uint64_t chunks = (n_triangles + kChunk - 1) / kChunk;
     const uint64_t p = chunks ? 256 / chunks : 1;
     return p < 1 ? 1u : (p > 32 ? 32u : (uint32_t)p);

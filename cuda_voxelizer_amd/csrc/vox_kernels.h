@@ -80,15 +80,17 @@ inline RegionKnobs region_knobs(unsigned flags) {
 }
 
 bool chunk_supported(uint32_t G);
-// Default algorithm: a mesh with few triangles for its grid has large voxel
-// boxes (and many columns) per triangle, which the output-tiled algorithm
-// spreads over the whole device; the chunk (surface) and marker (solid)
-// algorithms work one lane or one workgroup per triangle and are for meshes
-// whose triangles cover a few voxels each.  Measured on MI355X: 1,280
-// triangles at 256^3: surface tile 0.11 ms vs chunk 2.3 ms, solid tile 0.11
-// ms vs marker 0.27 ms; 1,048,576 triangles at 512^3: surface chunk 0.06 ms
-// vs tile 0.18 ms; 1,310,720 at 1024^3 solid: marker 0.15 ms vs tile 0.41 ms.
-inline bool prefer_tile_surface(uint64_t n_triangles, uint32_t G) {
+// A coarse mesh has few triangles for its grid, so each covers a large voxel
+// box and many columns.  Solid default: the output-tiled algorithm for a
+// coarse mesh, which spreads its columns over the whole device (1,280
+// triangles at 256^3: tile 0.09 ms vs marker 0.36 ms), the marker algorithm
+// otherwise (1,310,720 at 1024^3: marker 0.14 ms vs tile 0.41 ms).  Surface
+// default: chunk for every mesh; a coarse mesh's chunk splits its queued
+// items over up to 32 workgroups (chunk_parts), which measured faster than
+// tile from 6 to 1,310,720 triangles at 64^3 to 2048^3 (1,280 at 256^3: 0.057
+// vs 0.112 ms; a 12-triangle cube at 2048^3: 2.8 vs 8.4 ms), except for a few
+// hundred triangles at 128^3 to 512^3 (at most 1.5x slower).
+inline bool coarse_mesh(uint64_t n_triangles, uint32_t G) {
     return n_triangles < (uint64_t)G * G / 4u;
 }
 hipError_t launch_fill_zero(const vox_info& info, uint32_t* table, hipStream_t stream);
