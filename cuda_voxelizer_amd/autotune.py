@@ -1,9 +1,10 @@
 """Per-class algorithm autotuner for batched launches.
 
 A voxelization job's best algorithm (chunk / marker / tile / direct, all
-bit-identical) depends on its grid, layout, mode and size: chunk and marker
-win on fine meshes, tile on coarse ones (few large triangles), direct on fine
-solid meshes at small grids.  ``Tuner.choose`` times
+bit-identical) depends on its grid, layout, mode and size: chunk wins on most
+surfaces, coarse ones included (it splits their large triangles over several
+workgroups), marker on fine solid meshes and tile on coarse ones, direct on
+fine solid meshes at small grids.  ``Tuner.choose`` times
 every applicable algorithm once per class (grid, layout, surface/solid, log2
 size bucket) on the job it is asked about, keeps the fastest, and remembers it
 in a JSON cache keyed by the device name, so later jobs and later processes
