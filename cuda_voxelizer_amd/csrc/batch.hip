@@ -222,6 +222,7 @@ static vox_status plan_batch(const vox_batch_item* items, int count, vox_batch* 
         else if (algo & VOX_FLAG_ALGO_TILE) a = have_plan ? TILE : DIRECT;
         else if (algo & VOX_FLAG_ALGO_CHUNK) a = chunk_supported(G) ? CHUNK : DIRECT;
         else if (algo & VOX_FLAG_ALGO_MARKER) a = have_marker ? MARKER : (have_plan ? TILE : DIRECT);
+        else if (solid && prefer_direct_solid(it.info.n_triangles, G)) a = DIRECT;
         else if (solid && have_marker && !(have_plan && coarse_mesh(it.info.n_triangles, G))) a = MARKER;
         else if (solid) a = have_plan ? TILE : DIRECT;
         else a = chunk_supported(G) ? CHUNK : (have_plan ? TILE : DIRECT);

@@ -157,7 +157,8 @@ vox_status get_slab_workspace(hipStream_t st, const SlabPlan& P, uint64_t n, voi
 // plan for this grid/layout), chunk (surface only, LDS word merging + global
 // atomics after a fill), marker (solid) and direct (one triangle per lane).
 // Without an explicit flag: chunk for surface; tile for a coarse solid mesh,
-// marker for the others; direct when none fits.
+// direct for a very fine one on a small grid, marker for the others; direct
+// when none fits.
 vox_status run(const vox_info* info, const float* d_tris, uint32_t* d_table, int layout, unsigned flags,
                uint64_t* d_offrow, void* stream, bool solid) {
     vox_status s = validate(info, d_tris, d_table, layout);
@@ -189,6 +190,7 @@ vox_status run(const vox_info* info, const float* d_tris, uint32_t* d_table, int
     else if (flags & VOX_FLAG_ALGO_CHUNK) algo = chunk_supported(G) ? CHUNK : DIRECT;
     else if (flags & VOX_FLAG_ALGO_MARKER) algo = have_marker ? MARKER : (have_plan ? TILE : DIRECT);
     else if (solid && have_plan && coarse_mesh(info->n_triangles, G)) algo = TILE;
+    else if (solid && prefer_direct_solid(info->n_triangles, G)) algo = DIRECT;
     else if (solid) algo = have_marker ? MARKER : (have_plan ? TILE : DIRECT);
     else algo = chunk_supported(G) ? CHUNK : (have_plan ? TILE : DIRECT);
     hipError_t e;

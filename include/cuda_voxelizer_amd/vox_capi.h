@@ -60,8 +60,8 @@ typedef enum vox_status {
                                     (the zero-fill is fused into the kernels). */
 /* Algorithm selection (at most one; default: the library's choice per grid,
    layout and mode -- chunk for surface; for solid, tile when the mesh has fewer
-   than G^2/4 triangles (large triangles), marker otherwise).  All algorithms
-   give identical tables. */
+   than G^2/4 triangles (large triangles), direct when it has at least 4 G^2 at
+   G <= 64, marker otherwise).  All algorithms give identical tables. */
 #define VOX_FLAG_ALGO_DIRECT 0x2u /* one triangle per lane, global atomics per word */
 #define VOX_FLAG_ALGO_TILE 0x4u   /* triangles binned to output tiles; one workgroup
                                     builds each tile in LDS and writes it once */
