@@ -12,10 +12,10 @@
 // rest of their wave's) are queued; afterwards their owners publish test
 // records to LDS (up to kRecs per round) and the workgroup deals the rounds'
 // (row, 32-voxel x chunk) items evenly over its lanes, testing only the x
-// range that the plane band leaves open.  The table is then updated with one non-returning global
-// atomicOr per distinct word; words that do not fit in the hash go straight
-// to global atomics.  The table is zero-filled beforehand (unless
-// accumulating).
+// range that the plane band leaves open.  The table is then updated with one
+// non-returning global atomicOr per distinct word; words that do not fit in
+// the hash go straight to global atomics.  The table is zero-filled
+// beforehand (unless accumulating).
 #include "jobs.h"
 #include "raster.h"
 #include "region.h"
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(kChunk, LOOP ? 4 : (WIDE ? 6 : 5))
             const int ny = T.hi[1] - T.lo[1] + 1, nz = T.hi[2] - T.lo[2] + 1;
             const int64_t vol = (int64_t)nx * ny * nz;
             big = vol > (it.big ? (int64_t)it.big : (int64_t)kBigVox) || nx > 32;
-            rows = big ? 0 : ny * nz;  // below kBigVox here, so no overflow
+            rows = big ? 0 : ny * nz;  // at most the queue threshold here (<= 2^15): no overflow
         }
         if (!it.big && __ballot(rows > kRowsFloor)) {
             // sum over the wave (dead and queued lanes add 0); the mean is
